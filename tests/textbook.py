@@ -1,0 +1,241 @@
+"""Independent textbook formulations used to pin the oracle.
+
+These are written from the published methods, not from the reference code:
+  - physical Euler / ideal-MHD fluxes;
+  - HLLC in Toro's star-state form (Toro, Riemann Solvers and Numerical
+    Methods for Fluid Dynamics, 3rd ed., eqs. 10.33-10.39);
+  - HLLD of Miyoshi & Kusano (2005, J. Comput. Phys. 208, 315), eqs. 38-63;
+  - the exact Riemann solver for the Euler equations (Toro ch. 4);
+  - Voronov (1997) hydrogen ionisation fit and the radiative recombination
+    rate 2.6e-19 T_eV^-1/2 m^3/s.
+Primitive state order: (rho, vn, vt1, vt2, p[, Bn, Bt1, Bt2]).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def euler_cons(v, gm):
+    r, u, vv, w, p = v
+    return np.array([r, r * u, r * vv, r * w, p / (gm - 1) + 0.5 * r * (u * u + vv * vv + w * w)])
+
+
+def euler_flux(v, gm):
+    r, u, vv, w, p = v
+    E = p / (gm - 1) + 0.5 * r * (u * u + vv * vv + w * w)
+    return np.array([r * u, r * u * u + p, r * u * vv, r * u * w, u * (E + p)])
+
+
+def mhd_cons(v, gm):
+    r, u, vv, w, p, bx, by, bz = v[:8]
+    E = p / (gm - 1) + 0.5 * r * (u * u + vv * vv + w * w) + 0.5 * (bx * bx + by * by + bz * bz)
+    return np.array([r, r * u, r * vv, r * w, E, bx, by, bz])
+
+
+def mhd_flux(v, gm):
+    r, u, vv, w, p, bx, by, bz = v[:8]
+    b2 = bx * bx + by * by + bz * bz
+    pt = p + 0.5 * b2
+    E = p / (gm - 1) + 0.5 * r * (u * u + vv * vv + w * w) + 0.5 * b2
+    vb = u * bx + vv * by + w * bz
+    return np.array([r * u, r * u * u + pt - bx * bx, r * u * vv - bx * by, r * u * w - bx * bz,
+                     (E + pt) * u - bx * vb, 0.0, by * u - bx * vv, bz * u - bx * w])
+
+
+def toro_hllc(vl, vr, gm, acc=1.03):
+    """HLLC, star states of Toro eq. 10.39, wave speeds acc*min/max(u -+ c)."""
+    rl, ul, _, _, pl = vl
+    rr, ur, _, _, pr = vr
+    cl, cr = math.sqrt(gm * pl / rl), math.sqrt(gm * pr / rr)
+    sl = acc * min(ul - cl, ur - cr)
+    sr = acc * max(ul + cl, ur + cr)
+    ss = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur))
+    Ul, Ur = euler_cons(vl, gm), euler_cons(vr, gm)
+    Fl, Fr = euler_flux(vl, gm), euler_flux(vr, gm)
+
+    def star(v, U, s):
+        r, u, vv, w, p = v
+        f = r * (s - u) / (s - ss)
+        return f * np.array([1.0, ss, vv, w, U[4] / r + (ss - u) * (ss + p / (r * (s - u)))])
+
+    if sl >= 0:
+        return Fl
+    if sr <= 0:
+        return Fr
+    if ss >= 0:
+        return Fl + sl * (star(vl, Ul, sl) - Ul)
+    return Fr + sr * (star(vr, Ur, sr) - Ur)
+
+
+def fast_speed(v, gm):
+    r, u, vv, w, p, bx, by, bz = v[:8]
+    b2 = bx * bx + by * by + bz * bz
+    a2 = gm * p
+    return math.sqrt((a2 + b2 + math.sqrt((a2 + b2) ** 2 - 4 * a2 * bx * bx)) / (2 * r))
+
+
+def mk_hlld(vl, vr, gm):
+    """Miyoshi & Kusano (2005) HLLD for constant Bn (eqs. 38-63); returns the
+    flux and the name of the region sampled."""
+    bx = vl[5]
+    assert vl[5] == vr[5]
+    cfl, cfr = fast_speed(vl, gm), fast_speed(vr, gm)
+    sl = min(vl[1], vr[1]) - max(cfl, cfr)
+    sr = max(vl[1], vr[1]) + max(cfl, cfr)
+    Ul, Ur = mhd_cons(vl, gm), mhd_cons(vr, gm)
+    Fl, Fr = mhd_flux(vl, gm), mhd_flux(vr, gm)
+    if sl >= 0:
+        return Fl, "L"
+    if sr <= 0:
+        return Fr, "R"
+
+    def ptot(v):
+        return v[4] + 0.5 * (v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
+
+    rl, ul = vl[0], vl[1]
+    rr, ur = vr[0], vr[1]
+    sm = ((sr - ur) * rr * ur - (sl - ul) * rl * ul - ptot(vr) + ptot(vl)) / ((sr - ur) * rr - (sl - ul) * rl)
+    pts = ptot(vl) + rl * (sl - ul) * (sm - ul)
+
+    def star(v, U, s):
+        r, u, vv, w, p, _, by, bz = v[:8]
+        rs = r * (s - u) / (s - sm)
+        den = r * (s - u) * (s - sm) - bx * bx
+        vs = vv - bx * by * (sm - u) / den
+        ws = w - bx * bz * (sm - u) / den
+        bys = by * (r * (s - u) ** 2 - bx * bx) / den
+        bzs = bz * (r * (s - u) ** 2 - bx * bx) / den
+        vb = u * bx + vv * by + w * bz
+        vbs = sm * bx + vs * bys + ws * bzs
+        es = ((s - u) * U[4] - ptot(v) * u + pts * sm + bx * (vb - vbs)) / (s - sm)
+        return rs, vs, ws, bys, bzs, np.array([rs, rs * sm, rs * vs, rs * ws, es, bx, bys, bzs])
+
+    rsl, vsl, wsl, bysl, bzsl, Usl = star(vl, Ul, sl)
+    rsr, vsr, wsr, bysr, bzsr, Usr = star(vr, Ur, sr)
+    ssl = sm - abs(bx) / math.sqrt(rsl)
+    ssr = sm + abs(bx) / math.sqrt(rsr)
+    if ssl >= 0:
+        return Fl + sl * (Usl - Ul), "L*"
+    if ssr <= 0:
+        return Fr + sr * (Usr - Ur), "R*"
+    sgn = math.copysign(1.0, bx)
+    a, b = math.sqrt(rsl), math.sqrt(rsr)
+    v2 = (a * vsl + b * vsr + (bysr - bysl) * sgn) / (a + b)
+    w2 = (a * wsl + b * wsr + (bzsr - bzsl) * sgn) / (a + b)
+    by2 = (a * bysr + b * bysl + a * b * (vsr - vsl) * sgn) / (a + b)
+    bz2 = (a * bzsr + b * bzsl + a * b * (wsr - wsl) * sgn) / (a + b)
+    vb2 = sm * bx + v2 * by2 + w2 * bz2
+    if sm >= 0:
+        e2 = Usl[4] - a * ((sm * bx + vsl * bysl + wsl * bzsl) - vb2) * sgn
+        U2 = np.array([rsl, rsl * sm, rsl * v2, rsl * w2, e2, bx, by2, bz2])
+        return Fl + ssl * U2 - (ssl - sl) * Usl - sl * Ul, "L**"
+    e2 = Usr[4] + b * ((sm * bx + vsr * bysr + wsr * bzsr) - vb2) * sgn
+    U2 = np.array([rsr, rsr * sm, rsr * v2, rsr * w2, e2, bx, by2, bz2])
+    return Fr + ssr * U2 - (ssr - sr) * Usr - sr * Ur, "R**"
+
+
+def hlld_fallback_triggered(vl, vr, gm, factor=1.001):
+    """True when the HLLC-G fallback condition of the OpenMHD formulation
+    (S_L >= S_M - factor*S_A,L or S_M + factor*S_A,R >= S_R) holds."""
+    bx = vl[5]
+    cfl, cfr = fast_speed(vl, gm), fast_speed(vr, gm)
+    sl = min(vl[1], vr[1]) - max(cfl, cfr)
+    sr = max(vl[1], vr[1]) + max(cfl, cfr)
+    if sl >= 0 or sr <= 0:
+        return False
+
+    def ptot(v):
+        return v[4] + 0.5 * (v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
+    rl, ul, rr, ur = vl[0], vl[1], vr[0], vr[1]
+    sm = ((sr - ur) * rr * ur - (sl - ul) * rl * ul - ptot(vr) + ptot(vl)) / ((sr - ur) * rr - (sl - ul) * rl)
+    rsl = rl * (sl - ul) / (sl - sm)
+    rsr = rr * (sr - ur) / (sr - sm)
+    if rsl <= 0 or rsr <= 0:
+        return True
+    sal, sar = abs(bx) / math.sqrt(rsl), abs(bx) / math.sqrt(rsr)
+    return (sl >= sm - factor * sal) or (sm + factor * sar >= sr)
+
+
+# --------------------------------------------------------------------------
+def exact_riemann_euler(wl, wr, gm, xs, t, x0=0.5):
+    """Exact solution of the 1-D Euler Riemann problem (Toro ch. 4), sampled at
+    positions xs at time t.  wl, wr = (rho, u, p).  Returns rho, u, p arrays."""
+    rl, ul, pl = wl
+    rr, ur, pr = wr
+    cl, cr = math.sqrt(gm * pl / rl), math.sqrt(gm * pr / rr)
+    g1 = (gm - 1) / (2 * gm)
+    g2 = (gm + 1) / (2 * gm)
+
+    def fK(p, rk, pk, ck):
+        if p > pk:
+            ak = 2 / ((gm + 1) * rk)
+            bk = (gm - 1) / (gm + 1) * pk
+            q = math.sqrt(ak / (p + bk))
+            return (p - pk) * q, q * (1 - (p - pk) / (2 * (bk + p)))
+        pr_ = p / pk
+        return 2 * ck / (gm - 1) * (pr_ ** g1 - 1), 1 / (rk * ck) * pr_ ** (-g2)
+
+    p = max(1e-8, 0.5 * (pl + pr))
+    for _ in range(100):
+        fl, dl = fK(p, rl, pl, cl)
+        fr, dr = fK(p, rr, pr, cr)
+        dp = (fl + fr + ur - ul) / (dl + dr)
+        p = max(1e-10, p - dp)
+        if abs(dp) < 1e-15 * p:
+            break
+    fl, _ = fK(p, rl, pl, cl)
+    fr, _ = fK(p, rr, pr, cr)
+    us = 0.5 * (ul + ur) + 0.5 * (fr - fl)
+    out = np.zeros((3, len(xs)))
+    for n, x in enumerate(xs):
+        s = (x - x0) / t
+        if s <= us:  # left of contact
+            if p > pl:
+                rs = rl * ((p / pl + (gm - 1) / (gm + 1)) / ((gm - 1) / (gm + 1) * p / pl + 1))
+                ss = ul - cl * math.sqrt(g2 * p / pl + g1)
+                val = (rl, ul, pl) if s < ss else (rs, us, p)
+            else:
+                rs = rl * (p / pl) ** (1 / gm)
+                csl = cl * (p / pl) ** g1
+                if s < ul - cl:
+                    val = (rl, ul, pl)
+                elif s > us - csl:
+                    val = (rs, us, p)
+                else:
+                    u = 2 / (gm + 1) * (cl + (gm - 1) / 2 * ul + s)
+                    c = 2 / (gm + 1) * (cl + (gm - 1) / 2 * (ul - s))
+                    r = rl * (c / cl) ** (2 / (gm - 1))
+                    val = (r, u, pl * (c / cl) ** (1 / g1))
+        else:
+            if p > pr:
+                rs = rr * ((p / pr + (gm - 1) / (gm + 1)) / ((gm - 1) / (gm + 1) * p / pr + 1))
+                ss = ur + cr * math.sqrt(g2 * p / pr + g1)
+                val = (rr, ur, pr) if s > ss else (rs, us, p)
+            else:
+                rs = rr * (p / pr) ** (1 / gm)
+                csr = cr * (p / pr) ** g1
+                if s > ur + cr:
+                    val = (rr, ur, pr)
+                elif s < us + csr:
+                    val = (rs, us, p)
+                else:
+                    u = 2 / (gm + 1) * (-cr + (gm - 1) / 2 * ur + s)
+                    c = 2 / (gm + 1) * (cr - (gm - 1) / 2 * (ur - s))
+                    r = rr * (c / cr) ** (2 / (gm - 1))
+                    val = (r, u, pr * (c / cr) ** (1 / g1))
+        out[:, n] = val
+    return out
+
+
+def voronov_h(T_eV):
+    """Voronov (1997) collisional ionisation rate of H, m^3/s:
+    A (1+P U^1/2) U^K exp(-U) / (X+U), U = 13.6/T, A=2.91e-8 cm^3/s, P=0,
+    X=0.232, K=0.39."""
+    U = 13.6 / T_eV
+    return 2.91e-14 * U ** 0.39 * math.exp(-U) / (0.232 + U)
+
+
+def radiative_rec_h(T_eV):
+    return 2.6e-19 / math.sqrt(T_eV)
