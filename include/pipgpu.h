@@ -1,0 +1,113 @@
+/*
+ * pipgpu.h -- C-ABI of libpipgpu.so, the MI355X (gfx950) implementation of
+ * PIP's per-timestep two-fluid update (flag_eqs='PIP', HLL branch with HLLD
+ * for the plasma and HLLC for the neutrals, minmod MUSCL, SSP-RK, explicit
+ * ion-neutral collisions, set_IR case 3 ionisation/recombination, CFL dt).
+ *
+ * The reference exposes this path as Fortran module procedures operating on
+ * module globals; each entry point below names the procedure it replaces.
+ * Conventions:
+ *   - arrays are the reference's explicit-shape U(ix,jx,kx=1,nvar) in
+ *     column-major order (x fastest, variable slowest), ghosts included;
+ *   - every call returns 0 on success or a negative code; pip_last_error()
+ *     returns the message of the last failure on the calling thread;
+ *   - the state lives on the GPU between calls; host copies only happen in
+ *     pip_upload / pip_download (and the 8-byte dt of pip_cfl);
+ *   - calls on one context must come from one host thread at a time.
+ */
+#ifndef PIPGPU_H
+#define PIPGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIP_OK 0
+#define PIP_ERR_ARG -1
+#define PIP_ERR_HIP -2
+#define PIP_ERR_STATE -3
+#define PIP_ERR_UNSUPPORTED -4
+
+/* stop_sim flags (IO_rot.f90:625-674) */
+#define PIP_STOP_DT_SMALL 1
+#define PIP_STOP_NAN 2
+
+/* Configuration: the subset of module globalvar (Globalvar.f90:42-101) and
+ * of the derived values of get_parameters (Model_rot.f90:255-333) that the
+ * two-fluid step reads. */
+typedef struct pip_cfg {
+    int ndim;          /* 1 or 2 (kx = 1)                                 */
+    int ix, jx;        /* padded sizes, margin = 2 (jx = 1 when ndim = 1)  */
+    int nvar_m;        /* 8, or 9 with flag_divb = 1                       */
+    int flag_divb;     /* Dedner GLM cleaning (2-D only)                   */
+    int t_order;       /* SSP-RK order: 2 or 3 (flag_time = 1)             */
+    int flag_ir;       /* 0 off, 3 = Popescu et al. empirical rates        */
+    int flag_ir_type;  /* 0 (ion_pot/arb_heat), 1, 2                       */
+    int flag_col;      /* 1: constant alpha_c = col (parity used by IR)     */
+    int ac_field;      /* 1: per-cell alpha_c uploaded with the state      */
+    int flag_bnd[4];   /* x-, x+, y-, y+: 1 periodic, 2-7/11 mirror, 10 free */
+    double gm, safety, ro_lim, pr_lim, dx, dy;
+    double col, db_clean, t_ir, T0, n0, beta;
+} pip_cfg;
+
+typedef struct pip_ctx pip_ctx;
+
+/* allocate_vars + initialize_bnd (Model_rot.f90:376-392,
+ * Boundary_rot.f90:20-32): device state for one domain. */
+int pip_create(const pip_cfg *cfg, pip_ctx **out);
+int pip_destroy(pip_ctx *ctx);
+
+/* Host -> device copy of U_h(ix,jx,1,5), U_m(ix,jx,1,nvar_m) and, when
+ * cfg.ac_field = 1, ac(ix,jx,1) (Globalvar.f90:17,26). */
+int pip_upload(pip_ctx *ctx, const double *U_h, const double *U_m, const double *ac);
+/* Device -> host copy of the state (the arrays IO_rot.f90:362-374 writes). */
+int pip_download(pip_ctx *ctx, double *U_h, double *U_m);
+
+/* Prologue tail, Initial_rot.f90:140-160: ghost fill, set_coefficients(.,0)
+ * (set_IR incl. arb_heat), cfl(dt); followed by the first pass of the time
+ * loop's set_IR/cfl (main.f90:32-36). Returns the first dt. */
+int pip_prologue(pip_ctx *ctx, double *dt);
+
+/* set_IR (PIP_rot.f90:127-311, case 3): get_Te rewrites + Gm_ion, Gm_rec. */
+int pip_set_ir(pip_ctx *ctx);
+/* cfl (Scheme_rot.f90:183-213): rewrites U, returns dt; when into_global = 1
+ * the context's dt and cmax are updated as `call cfl(U_h,U_m,dt)` does. */
+int pip_cfl(pip_ctx *ctx, double *dt, int into_global);
+/* run_solver (Solver_rot.f90:42-81) with the context's dt. */
+int pip_run_solver(pip_ctx *ctx);
+/* PIPbnd (Boundary_rot.f90:34-52) single rank. */
+int pip_bnd(pip_ctx *ctx);
+
+/* The whole time loop of main.f90:31-68 on the device, fused: runs until
+ * time >= tend, max_steps steps, or a stop_sim condition.  steps_done and
+ * time may be NULL. */
+int pip_advance(pip_ctx *ctx, int max_steps, double tend, int *steps_done, double *time);
+/* Enqueue n fused steps without waiting (no host sync); for timing loops. */
+int pip_enqueue_steps(pip_ctx *ctx, int n, double tend);
+int pip_sync(pip_ctx *ctx);
+
+int pip_get_scalars(pip_ctx *ctx, double *dt, double *time, double *cmax, long *nt);
+int pip_set_time(pip_ctx *ctx, double time, double dt);
+/* stop_sim flags of the last step (PIP_STOP_*). */
+int pip_status(pip_ctx *ctx, int *flags);
+
+/* Rate fields after pip_set_ir (Gm_rec, Gm_ion, arb_heat), for inspection. */
+int pip_get_rates(pip_ctx *ctx, double *gm_rec, double *gm_ion, double *arb_heat);
+
+/* Device pointers and stream for zero-copy interop (torch.autograd wrapper,
+ * ensemble driver).  U: nvar_m + 5 planes of ix*jx doubles (U_m then U_h). */
+int pip_device_state(pip_ctx *ctx, double **U, double **ac);
+int pip_get_stream(pip_ctx *ctx, void **stream);
+/* Stage-kernel tile shape (autotuner): tx*ty threads per workgroup. */
+int pip_set_tile(pip_ctx *ctx, int tx, int ty);
+/* Average device time (ms) of the last timed enqueue of the dominant kernel;
+ * measured with hipEvents on the context's stream. */
+int pip_time_steps(pip_ctx *ctx, int nsteps, double *ms_total, double *ms_stage1, double *ms_stage2);
+
+const char *pip_last_error(void);
+const char *pip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -64,6 +64,11 @@ def lib():
         L.orc_step.restype = ctypes.c_int
         L.orc_stop_sim.argtypes = [ctypes.c_void_p]
         L.orc_stop_sim.restype = ctypes.c_int
+        for name in ("orc_exp", "orc_log"):
+            getattr(L, name).argtypes = [ctypes.c_double]
+            getattr(L, name).restype = ctypes.c_double
+        L.orc_pow.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_pow.restype = ctypes.c_double
         L.orc_minmod.argtypes = [ctypes.c_double] * 3
         L.orc_minmod.restype = ctypes.c_double
         L.orc_hllc_flux.argtypes = [P, P, ctypes.c_double, P]

@@ -41,6 +41,48 @@ struct orc_state {
 };
 
 static double dmin(double a, double b) { return (a < b) ? a : b; }
+
+/* exp / log / pow of the reference's dexp and ** (PIP_rot.f90:216,
+ * Scheme_rot.f90:552), evaluated as fixed sequences of IEEE additions and
+ * multiplications so that the HIP path can reproduce them bit for bit; they
+ * are within a few ulp of the C library's (tested).
+ * exp: Cody-Waite reduction x = k ln2 + r, degree-13 Taylor, scale 2^k.
+ * log: x = m 2^e, m in [1/sqrt2, sqrt2), 2 atanh((m-1)/(m+1)) to s^25. */
+#define LN2_HI 6.93147180369123816490e-01
+#define LN2_LO 1.90821492927058770002e-10
+#define INV_LN2 1.44269504088896338700e+00
+double orc_exp(double x) {
+    if (!(x == x)) return x;
+    if (x > 709.782712893383973096) return HUGE_VAL;
+    if (x < -745.133219101941108420) return 0.0;
+    double kd = rint(x * INV_LN2);
+    double r = (x - kd * LN2_HI) - kd * LN2_LO;
+    static const double c[14] = {1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0,
+                                 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+                                 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
+    double q = c[0];
+    for (int n = 1; n < 14; n++) q = q * r + c[n];
+    return ldexp(q, (int)kd);
+}
+double orc_log(double x) {
+    if (!(x > 0.0)) return (x == 0.0) ? -HUGE_VAL : (x - x) / (x - x);
+    if (x == HUGE_VAL) return x;
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double t = 1.0 / 25.0;
+    for (int k = 23; k >= 3; k -= 2) t = t * s2 + 1.0 / (double)k;
+    t = t * s2 + 1.0;
+    double lm = (2.0 * s) * t;
+    double ed = (double)e;
+    return ed * LN2_HI + (ed * LN2_LO + lm);
+}
+double orc_pow(double x, double y) {
+    if (x == 0.0) return (y > 0.0) ? 0.0 : HUGE_VAL;
+    return orc_exp(y * orc_log(x));
+}
 static double dmax(double a, double b) { return (a > b) ? a : b; }
 
 /* ---------------------------------------------------------------- set-up */
@@ -665,7 +707,7 @@ void orc_set_ir(orc_state *s) {
     double Te_0 = c->T0 / SGL(1.1604e4);
     double rec_fac = SGL(2.6e-19) * (c->n0 * SGL(1.0e6)) / sqrt(Te_0);
     double ioneq = (SGL(2.6e-19) / sqrt(Te_0))
-                   / (SGL(2.91e-14) / (0.232 + 13.6 / Te_0) * pow(13.6 / Te_0, 0.39) * exp(-13.6 / Te_0));
+                   / (SGL(2.91e-14) / (0.232 + 13.6 / Te_0) * orc_pow(13.6 / Te_0, 0.39) * orc_exp(-13.6 / Te_0));
     double f_n = ioneq / (ioneq + 1.0);
     double f_p = 1.0 - f_n;
     double f_p_p = 2.0 * f_p / (f_n + 2.0 * f_p);
@@ -676,7 +718,7 @@ void orc_set_ir(orc_state *s) {
         double rho = s->Um[k];
         s->gm_rec[k] = rho / sqrt(Te_p[k]) * c->t_ir / f_p * sqrt(tfac);
         double a = 13.6 / Te_0 / Te_p[k] * tfac;
-        double g = SGL(2.91e-14) * (c->n0 * SGL(1.0e6)) * rho * exp(-a) * pow(a, 0.39);
+        double g = SGL(2.91e-14) * (c->n0 * SGL(1.0e6)) * rho * orc_exp(-a) * orc_pow(a, 0.39);
         s->gm_ion[k] = g / (0.232 + a) / rec_fac / f_p * c->t_ir;
     }
     if (!s->arb_set) { /* first call only: nout==0 .and. flag_restart==-1 (:219-229) */
@@ -883,7 +925,7 @@ void orc_run_solver(orc_state *s) {
         add_flux_ssprk(s, s->Fm, s->Um0, s->Um, nv, dt_sub, c0, c1, istep);
         for (int k = 0; k < N * nv; k++) s->Um[k] = s->Um[k] + s->Sm[k] * dt_sub;
         if (c->flag_divb == 1) { /* divb_cleaning_Dedner_source Scheme_rot.f90:546-554 */
-            double f = exp(-dt_sub / s->dt * c->safety * c->db_clean);
+            double f = orc_exp(-dt_sub / s->dt * c->safety * c->db_clean);
             for (int k = 0; k < N; k++) s->Um[8 * N + k] = s->Um[8 * N + k] * f;
         }
         orc_pipbnd(s); /* PIPbnd(U_h,U_m,istep) :70 */

@@ -62,6 +62,9 @@ int orc_stop_sim(const orc_state *s);          /* IO_rot.f90:625-674       */
 void orc_set_fixed_rates(orc_state *s, int on);
 
 /* point functions (pure) */
+double orc_exp(double x);
+double orc_log(double x);
+double orc_pow(double x, double y);
 double orc_minmod(double u1, double u2, double u3);        /* Util_rot:59-76 */
 void orc_hllc_flux(const double *vl, const double *vr, double gm, double *flx);
 void orc_hlld_flux(const double *vl, const double *vr, int nvar, int divb,

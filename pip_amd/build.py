@@ -1,0 +1,76 @@
+"""Build recipes for the native parts of pip_amd (run on the CPU; hipcc
+cross-compiles for gfx950).
+
+  libpipgpu.so      HIP kernels + C-ABI (include/pipgpu.h)
+  pip_driver        Fortran driver through the iso_c_binding module
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libpipgpu.so")
+FDIR = os.path.join(HERE, "fortran")
+FBIN = os.path.join(LIBDIR, "pip_driver")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
+    # keep the compiler from emitting packed FP32 ops (not used on this path)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no FMA contraction: every product and sum is rounded separately, as in
+    # the reference built for x86-64; this makes the kernels reproduce the
+    # reference arithmetic bit for bit (the HLLC star state at
+    # discontinuities amplifies a contracted rounding to ~1e-6 relative)
+    "-ffp-contract=off",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout[-4000:])
+    return r.stdout
+
+
+def build_lib() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-I" + os.path.join(ROOT, "include"), "-o", tmp,
+          os.path.join(CSRC, "pipgpu.hip")])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran() -> str | None:
+    """Compile the iso_c_binding module and the Fortran driver against
+    libpipgpu.so; skipped (returns None) if flang is not installed."""
+    if not os.path.exists(FLANG) and shutil.which("flang") is None:
+        return None
+    flang = FLANG if os.path.exists(FLANG) else shutil.which("flang")
+    os.makedirs(LIBDIR, exist_ok=True)
+    mod_dir = os.path.join(LIBDIR, "fmod")
+    os.makedirs(mod_dir, exist_ok=True)
+    srcs = [os.path.join(FDIR, "pipgpu_mod.f90"), os.path.join(FDIR, "pip_driver.f90")]
+    _run([flang, "-O2", "-module-dir", mod_dir, *srcs, "-L" + LIBDIR, "-lpipgpu",
+          "-Wl,-rpath,$ORIGIN", "-o", FBIN])
+    return FBIN
+
+
+def build_all(fortran: bool = True):
+    out = {"lib": build_lib()}
+    if fortran and os.path.exists(os.path.join(FDIR, "pip_driver.f90")):
+        out["fortran"] = build_fortran()
+    return out
+
+
+if __name__ == "__main__":
+    print(build_all())

@@ -1,0 +1,600 @@
+// pip_kernels.h -- HIP kernels of the fused two-fluid step for gfx950.
+//
+// Device layout (identical to the reference arrays, so uploads are memcpy):
+// one buffer of NVT = nvar_m + 5 planes, plane n holding ix*jx doubles
+// (x fastest); planes 0..nvar_m-1 are U_m, planes nvar_m.. are U_h.
+//
+// One launch of k_stage performs one SSP-RK stage for both fluids on a
+// TX x TY tile of interior cells per workgroup:
+//   load  : (TX+4)x(TY+4) cells -> primitives in LDS (u2v_*, HLL_rot.f90:1136-1186)
+//   x     : (TX+1)*TY faces: minmod MUSCL + HLLD + HLLC -> flux tile in LDS
+//   cell  : source_PIP + SSP blend + x flux divergence (Solver_rot.f90:464-478)
+//   y     : TX*(TY+1) faces (permuted HLLD/HLLC) -> flux tile in LDS
+//   cell  : y flux divergence, add_source, Dedner psi damping, then (last
+//           stage of a fused step) the set_IR/cfl rewrite chain of main.f90
+//           with the dt reduction, and the store of the cell and its ghost
+//           images (Boundary_rot.f90:323-402 applied by index mapping).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pip_phys.h"
+
+namespace pip {
+
+// Reduction slots (bit patterns of non-negative doubles; min/max as uint64).
+enum { RED_DT = 0, RED_RHOAC = 1, RED_GMAX = 2, RED_IONPOT = 3, RED_NAN = 4, RED_N = 5 };
+
+struct DevScalars {
+  double dt;        // dt of the current step (global dt)
+  double time;
+  double cmax;      // GLM ch (Scheme_rot.f90:205-211)
+  double tend;
+  double dt_out;    // last cfl result
+  unsigned long long red[RED_N];
+  long long nt;
+  long long nmax;
+  int active;       // 1 while time < tend and no stop condition
+  int last;         // current step is the last one (skip the look-ahead rewrite)
+  int status;       // stop_sim flags
+  int ion_pot_valid;
+  int pad[4];
+};
+
+// Boundary description: per side type (1 periodic, 2 mirror, 10 free) and
+// sign masks (bit n set -> variable n flips sign), Boundary_rot.f90:239-320.
+struct Bnd {
+  int type[4];
+  unsigned sym[4];  // bits over NVT variables (U_m planes then U_h planes)
+};
+
+struct StageArgs {
+  const double* Uin;  // stage input (stencil)
+  const double* U0;   // state at run_solver entry (pointwise; may alias Uout)
+  double* Uout;
+  const double* ac;       // per-cell alpha_c or nullptr
+  const double* arb_heat; // IR type 0
+  double* ion_pot_out;    // nullable: ion_pot of this stage
+  DevScalars* sc;
+  int ix, jx, nx, ny;
+  long long N;            // plane stride ix*jx
+  int istep, t_order, epi;  // epi: 0 none, 1 one cfl round, 2 two rounds + dt reduction
+  double col;
+  double gmin_safety;     // safety*min(dx,dy,dz)
+  double dxyz_min;        // min(dx,dy,dz)
+  double dx, dy;
+  double db_clean;
+  Phys ph;
+  IRConst irc;
+  Bnd bnd;
+};
+
+PIP_HD unsigned long long as_u64(double x) {
+  union { double d; unsigned long long u; } c;
+  c.d = x;
+  return c.u;
+}
+PIP_HD double as_f64(unsigned long long x) {
+  union { double d; unsigned long long u; } c;
+  c.u = x;
+  return c.d;
+}
+
+// Stage coefficients: Solver_rot.f90:223-231 (float() conversions kept).
+__device__ __forceinline__ void stage_coeffs(int istep, int t_order, double dt, double& c0, double& c1,
+                                             double& dt_sub) {
+  if (istep == 1) {
+    c1 = 1.0; c0 = 1.0; dt_sub = dt;
+  } else {
+    c1 = (double)((float)(istep - 1) / (float)(2 * t_order - istep));
+    c0 = 1.0 - c1;
+    dt_sub = c1 * dt;
+  }
+}
+
+// Ghost images of interior index i along one direction of n cells padded by
+// 2 (C indices 2..n+1 interior): returns the count and fills dst[], with the
+// sign flag (mirror types flip the masked variables).
+__device__ __forceinline__ int ghost_images(int i, int npad, int tlo, int thi, int* dst, bool* flip) {
+  int k = 0;
+  // lower side
+  if (tlo == 1) {
+    if (i == npad - 4) { dst[k] = 0; flip[k++] = false; }
+    if (i == npad - 3) { dst[k] = 1; flip[k++] = false; }
+  } else if (tlo == 2) {
+    if (i == 3) { dst[k] = 0; flip[k++] = true; }
+    if (i == 2) { dst[k] = 1; flip[k++] = true; }
+  } else if (tlo == 10) {
+    if (i == 2) { dst[k] = 0; flip[k++] = false; dst[k] = 1; flip[k++] = false; }
+  }
+  // upper side
+  if (thi == 1) {
+    if (i == 2) { dst[k] = npad - 2; flip[k++] = false; }
+    if (i == 3) { dst[k] = npad - 1; flip[k++] = false; }
+  } else if (thi == 2) {
+    if (i == npad - 3) { dst[k] = npad - 2; flip[k++] = true; }
+    if (i == npad - 4) { dst[k] = npad - 1; flip[k++] = true; }
+  } else if (thi == 10) {
+    if (i == npad - 3) { dst[k] = npad - 2; flip[k++] = false; dst[k] = npad - 1; flip[k++] = false; }
+  }
+  return k;
+}
+
+// Store one cell value set (NVT vars) at (ci,cj) and at all its ghost images.
+template <int NDIM, int NVT>
+__device__ __forceinline__ void store_cell(double* U, long long N, int ix, int jx, int ci, int cj,
+                                           const double* u, const Bnd& b) {
+#pragma unroll
+  for (int n = 0; n < NVT; n++) U[n * N + (long long)cj * ix + ci] = u[n];
+  int xd[4], yd[4];
+  bool xf[4], yf[4];
+  int nxi = ghost_images(ci, ix, b.type[0], b.type[1], xd, xf);
+  int nyi = 0;
+  if (NDIM == 2) nyi = ghost_images(cj, jx, b.type[2], b.type[3], yd, yf);
+  if (nxi == 0 && nyi == 0) return;
+  // x images in the own row
+  for (int a = 0; a < nxi; a++) {
+    unsigned m = xf[a] ? b.sym[(xd[a] < 2) ? 0 : 1] : 0u;
+    for (int n = 0; n < NVT; n++)
+      U[n * N + (long long)cj * ix + xd[a]] = ((m >> n) & 1u) ? -u[n] : u[n];
+  }
+  if (NDIM == 2) {
+    for (int c = 0; c < nyi; c++) {
+      unsigned my = yf[c] ? b.sym[(yd[c] < 2) ? 2 : 3] : 0u;
+      for (int n = 0; n < NVT; n++)
+        U[n * N + (long long)yd[c] * ix + ci] = ((my >> n) & 1u) ? -u[n] : u[n];
+      for (int a = 0; a < nxi; a++) {
+        unsigned mx = xf[a] ? b.sym[(xd[a] < 2) ? 0 : 1] : 0u;
+        unsigned m = mx ^ my;
+        for (int n = 0; n < NVT; n++)
+          U[n * N + (long long)yd[c] * ix + xd[a]] = ((m >> n) & 1u) ? -u[n] : u[n];
+      }
+    }
+  }
+}
+
+// One round of the set_IR/cfl chain on one cell (main.f90:32-36 / :44-47):
+// set_IR rewrites (+ rates), cfl rewrites and the cell's dt terms.
+struct RoundOut {
+  double dtc, rhoac, gmax, ratio;
+};
+
+template <int NV, bool IR>
+__device__ __forceinline__ RoundOut cfl_round(double* um, double* uh, double ac, double ion_pot,
+                                              const StageArgs& a) {
+  RoundOut r;
+  double gr = 0.0, gi = 0.0;
+  if (IR) set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
+  r.dtc = cfl_cell<NV>(um, uh, a.gmin_safety, a.ph);
+  r.rhoac = dmax(um[0] * ac, uh[0] * ac);
+  r.gmax = dmax(gr, gi);
+  r.ratio = 0.0;
+  if (IR && a.ph.flag_ir_type == 0) {
+    double pr, te;
+    getpr_mhd(um, pr, te, a.ph);  // cfl_pip_ir :321
+    r.ratio = ion_pot / (pr / (a.ph.gm - 1.0));
+  }
+  return r;
+}
+
+// Workgroup min/max reduction of the dt terms, then one atomic per slot.
+template <int NT>
+__device__ __forceinline__ void block_reduce_store(RoundOut r, int nanflag, bool valid, DevScalars* sc,
+                                                   double* red_lds) {
+  double v0 = valid ? r.dtc : 1.0e300;
+  double v1 = valid ? r.rhoac : 0.0;
+  double v2 = valid ? r.gmax : 0.0;
+  double v3 = valid ? r.ratio : 0.0;
+  int fl = valid ? nanflag : 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 = dmin(v0, __shfl_xor(v0, off, 64));
+    v1 = dmax(v1, __shfl_xor(v1, off, 64));
+    v2 = dmax(v2, __shfl_xor(v2, off, 64));
+    v3 = dmax(v3, __shfl_xor(v3, off, 64));
+    fl |= __shfl_xor(fl, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int NW = (NT + 63) / 64;
+  __syncthreads();
+  if (lane == 0) {
+    red_lds[wid * 5 + 0] = v0;
+    red_lds[wid * 5 + 1] = v1;
+    red_lds[wid * 5 + 2] = v2;
+    red_lds[wid * 5 + 3] = v3;
+    red_lds[wid * 5 + 4] = (double)fl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NW; w++) {
+      v0 = dmin(v0, red_lds[w * 5 + 0]);
+      v1 = dmax(v1, red_lds[w * 5 + 1]);
+      v2 = dmax(v2, red_lds[w * 5 + 2]);
+      v3 = dmax(v3, red_lds[w * 5 + 3]);
+      fl |= (int)red_lds[w * 5 + 4];
+    }
+    atomicMin(&sc->red[RED_DT], as_u64(v0));
+    atomicMax(&sc->red[RED_RHOAC], as_u64(v1));
+    atomicMax(&sc->red[RED_GMAX], as_u64(v2));
+    atomicMax(&sc->red[RED_IONPOT], as_u64(v3));
+    if (fl) atomicOr(&sc->red[RED_NAN], 1ull);
+  }
+}
+
+// ------------------------------------------------------------ stage kernel
+template <int NDIM, int NV, bool IR, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
+  constexpr int NVT = NV + 5;
+  constexpr int NT = TX * TY;
+  constexpr int HX = TX + 4;
+  constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
+  constexpr int OY = (NDIM == 2) ? 2 : 0;
+  constexpr int NFX = (TX + 1) * TY;
+  constexpr int NFY = (NDIM == 2) ? TX * (TY + 1) : 0;
+  constexpr int NF = (NFX > NFY) ? NFX : NFY;
+  extern __shared__ double smem[];
+  double* V = smem;                      // NVT * HY * HX
+  double* F = smem + NVT * HY * HX;      // NVT * NF
+
+  DevScalars* sc = a.sc;
+  if (!sc->active) return;
+  const double dt = sc->dt;
+  const double cmax = sc->cmax;
+  double c0, c1, dt_sub;
+  stage_coeffs(a.istep, a.t_order, dt, c0, c1, dt_sub);
+
+  const int tid = threadIdx.x;
+  const int ntx = (a.nx + TX - 1) / TX;
+  const int bid = blockIdx.x;
+  const int bxi = bid % ntx, byi = bid / ntx;
+  const int i0 = bxi * TX;  // first interior column of the tile (interior coords)
+  const int j0 = byi * TY;
+  const long long N = a.N;
+  const int ix = a.ix;
+
+  // ---- load halo tile -> primitives in LDS
+  for (int c = tid; c < HX * HY; c += NT) {
+    int hx = c % HX, hy = c / HX;
+    int gi = i0 + hx, gj = (NDIM == 2) ? j0 + hy : 0;
+    if (gi < a.ix && gj < a.jx) {
+      long long o = (long long)gj * ix + gi;
+      double um[NV], uh[5];
+#pragma unroll
+      for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + o];
+#pragma unroll
+      for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + o];
+      if (IR) {  // set_IR of set_coefficients(.,1) rewrites E (Solver_rot.f90:111-113)
+        double pr, te;
+        getpr_hd(uh, pr, te, a.ph);
+        getpr_mhd(um, pr, te, a.ph);
+      }
+      double vm[NV], vh[5];
+      u2v_mhd<NV>(um, vm, a.ph);
+      u2v_hd(uh, vh, a.ph);
+#pragma unroll
+      for (int n = 0; n < NV; n++) V[(n * HY + hy) * HX + hx] = vm[n];
+#pragma unroll
+      for (int n = 0; n < 5; n++) V[((NV + n) * HY + hy) * HX + hx] = vh[n];
+    }
+  }
+  __syncthreads();
+
+  // ---- x faces: face fi (0..TX) between tile cells fi-1 and fi
+  for (int f = tid; f < NFX; f += NT) {
+    int fi = f % (TX + 1), fj = f / (TX + 1);
+    if (i0 + fi > a.nx || j0 + fj >= a.ny) continue;
+    int hy = fj + OY;
+    const double* row = V + hy * HX;
+    double vl[NVT], vr[NVT];
+#pragma unroll
+    for (int n = 0; n < NVT; n++) {
+      const double* r = row + n * HY * HX + fi;
+      double a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+      vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+      vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+    }
+    vl[0] = dmax(vl[0], a.ph.ro_lim);
+    vr[0] = dmax(vr[0], a.ph.ro_lim);
+    vl[NV] = dmax(vl[NV], a.ph.ro_lim);
+    vr[NV] = dmax(vr[NV], a.ph.ro_lim);
+    double fl[NVT];
+    hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+    hllc_flux(vl + NV, vr + NV, a.ph.gm, fl + NV);
+#pragma unroll
+    for (int n = 0; n < NVT; n++) F[n * NF + f] = fl[n];
+  }
+  __syncthreads();
+
+  // ---- own cell: sources and x update
+  const int ti = tid % TX, tj = tid / TX;
+  const int ci = i0 + ti + 2, cj = (NDIM == 2) ? j0 + tj + 2 : 0;
+  const bool valid = (i0 + ti < a.nx) && (j0 + tj < a.ny);
+  double un[NVT];
+  double Sm[5], Sh[5];
+  double ion_pot = 0.0;
+  double acc = a.col;
+  const long long oc = (long long)cj * ix + ci;
+  if (valid) {
+    double um[NV], uh[5];
+#pragma unroll
+    for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + oc];
+#pragma unroll
+    for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + oc];
+    if (a.ac) acc = a.ac[oc];
+    double gr = 0.0, gi = 0.0, arb = 0.0;
+    if (IR) {
+      set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
+      if (a.ph.flag_ir_type == 0) arb = a.arb_heat[oc];
+    }
+    source_pip(um, uh, acc, gr, gi, arb, a.ph, a.irc.heat_fac, Sh, Sm, ion_pot);
+    if (a.ion_pot_out) a.ion_pot_out[oc] = ion_pot;
+    const double ax = dt_sub / a.dx;
+    const int fL = tj * (TX + 1) + ti;
+    if (a.istep == 1) {
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        double u0 = a.U0[n * N + oc];
+        un[n] = u0 - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        double u0 = a.U0[n * N + oc];
+        double ur = (n < NV) ? um[n] : uh[n - NV];
+        un[n] = c0 * u0 + c1 * ur - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+      }
+    }
+  }
+
+  // ---- y faces
+  if (NDIM == 2) {
+    __syncthreads();
+    for (int f = tid; f < NFY; f += NT) {
+      int fi = f % TX, fj = f / TX;
+      if (i0 + fi >= a.nx || j0 + fj > a.ny) continue;
+      int hx = fi + 2;
+      double vl[NVT], vr[NVT];
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        // permutation (1,3,2,4,5,7,6,8,9)/(1,3,2,4,5): HLL_rot.f90:414-434, 315-325
+        int src = n;
+        if (n == 1) src = 2; else if (n == 2) src = 1;
+        else if (n == 5) src = 6; else if (n == 6) src = 5;
+        else if (n == NV + 1) src = NV + 2; else if (n == NV + 2) src = NV + 1;
+        const double* col = V + src * HY * HX + hx;
+        double a0 = col[(fj + 0) * HX], a1 = col[(fj + 1) * HX], a2 = col[(fj + 2) * HX],
+               a3 = col[(fj + 3) * HX];
+        vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+        vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+      }
+      vl[0] = dmax(vl[0], a.ph.ro_lim);
+      vr[0] = dmax(vr[0], a.ph.ro_lim);
+      vl[NV] = dmax(vl[NV], a.ph.ro_lim);
+      vr[NV] = dmax(vr[NV], a.ph.ro_lim);
+      double fl[NVT];
+      hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+      hllc_flux(vl + NV, vr + NV, a.ph.gm, fl + NV);
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        int dst = n;
+        if (n == 1) dst = 2; else if (n == 2) dst = 1;
+        else if (n == 5) dst = 6; else if (n == 6) dst = 5;
+        else if (n == NV + 1) dst = NV + 2; else if (n == NV + 2) dst = NV + 1;
+        F[dst * NF + f] = fl[n];
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      const double ay = dt_sub / a.dy;
+      const int fB = tj * TX + ti;
+#pragma unroll
+      for (int n = 0; n < NVT; n++) un[n] = un[n] - ay * (F[n * NF + fB + TX] - F[n * NF + fB]);
+    }
+  }
+
+  RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
+  int nanflag = 0;
+  if (valid) {
+    // add_source (Solver_rot.f90:498-503) for the five fluid variables
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      un[n] = un[n] + Sm[n] * dt_sub;
+      un[NV + n] = un[NV + n] + Sh[n] * dt_sub;
+    }
+    if (NV == 9) {  // divb_cleaning_Dedner_source (Scheme_rot.f90:546-554)
+      double fac = pexp(-dt_sub / dt * a.ph.safety * a.db_clean);
+      un[8] = un[8] * fac;
+    }
+    if (a.epi >= 1) {
+      double* um = un;
+      double* uh = un + NV;
+      r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
+      if (a.epi >= 2 && !sc->last) r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
+#pragma unroll
+      for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
+    }
+    store_cell<NDIM, NVT>(a.Uout, N, ix, a.jx, ci, cj, un, a.bnd);
+  }
+  if (a.epi >= 1) {
+    __syncthreads();
+    block_reduce_store<NT>(r, nanflag, valid, sc, F);
+  }
+}
+
+// ------------------------------------------------------- pointwise kernels
+// Rewrite chain on every cell (ghosts included, as the reference's array
+// statements do): flags select set_IR (+rates/arb_heat) and cfl.
+struct RoundArgs {
+  double* U;
+  const double* ac;
+  double* gm_rec;
+  double* gm_ion;
+  double* arb_heat;
+  const double* ion_pot;
+  DevScalars* sc;
+  int ix, jx, nx, ny;
+  long long N;
+  int do_ir, init_arb, do_cfl, read_rates;
+  double col, gmin_safety;
+  Phys ph;
+  IRConst irc;
+};
+
+template <int NV, bool IR>
+__global__ void __launch_bounds__(256) k_round(RoundArgs a) {
+  constexpr int NVT = NV + 5;
+  __shared__ double red_lds[4 * 5];
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = a.N;
+  bool inb = k < N;
+  RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
+  bool interior = false;
+  int nanflag = 0;
+  if (inb) {
+    int i = (int)(k % a.ix), j = (int)(k / a.ix);
+    interior = (i >= 2 && i < a.ix - 2) && (a.jx == 1 || (j >= 2 && j < a.jx - 2));
+    double u[NVT];
+#pragma unroll
+    for (int n = 0; n < NVT; n++) u[n] = a.U[n * N + k];
+    double* um = u;
+    double* uh = u + NV;
+    double acc = a.ac ? a.ac[k] : a.col;
+    double gr = 0.0, gi = 0.0;
+    if (IR && a.do_ir) {
+      set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
+      a.gm_rec[k] = gr;
+      a.gm_ion[k] = gi;
+      if (a.init_arb) a.arb_heat[k] = gi * uh[0] * a.irc.heat_fac;
+    } else if (IR && a.read_rates) {
+      gr = a.gm_rec[k];
+      gi = a.gm_ion[k];
+    }
+    if (a.do_cfl) {
+      double dtc = cfl_cell<NV>(um, uh, a.gmin_safety, a.ph);
+      r.dtc = interior ? dtc : 1.0e300;
+      r.rhoac = dmax(um[0] * acc, uh[0] * acc);
+      r.gmax = dmax(gr, gi);
+      if (IR && a.ph.flag_ir_type == 0) {
+        double pr, te;
+        getpr_mhd(um, pr, te, a.ph);
+        r.ratio = (a.ion_pot && interior) ? a.ion_pot[k] / (pr / (a.ph.gm - 1.0)) : 0.0;
+      }
+      if (interior) {
+#pragma unroll
+        for (int n = 0; n < NV; n++) nanflag |= !(u[n] * 0.0 == 0.0);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NVT; n++) a.U[n * N + k] = u[n];
+  }
+  if (a.do_cfl) block_reduce_store<256>(r, nanflag, inb, a.sc, red_lds);
+}
+
+// Ghost fill by the composite x-then-y map (Boundary_rot.f90:34-125).
+__device__ __forceinline__ int src_index(int g, int npad, int tlo, int thi, bool& flip) {
+  flip = false;
+  if (g < 2) {
+    if (tlo == 1) return npad - 4 + g;
+    if (tlo == 2) { flip = true; return 3 - g; }
+    return 2;
+  }
+  if (g >= npad - 2) {
+    if (thi == 1) return g - (npad - 4);
+    if (thi == 2) { flip = true; return 2 * npad - 5 - g; }
+    return npad - 3;
+  }
+  return g;
+}
+
+__global__ void __launch_bounds__(256) k_bnd(double* U, int nvt, int ix, int jx, long long N, Bnd b,
+                                             int ndim) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  int i = (int)(k % ix), j = (int)(k / ix);
+  bool gx = (i < 2 || i >= ix - 2);
+  bool gy = (ndim == 2) && (j < 2 || j >= jx - 2);
+  if (!gx && !gy) return;
+  bool fx = false, fy = false;
+  int si = gx ? src_index(i, ix, b.type[0], b.type[1], fx) : i;
+  int sj = gy ? src_index(j, jx, b.type[2], b.type[3], fy) : j;
+  unsigned m = 0;
+  if (fx) m ^= b.sym[(i < 2) ? 0 : 1];
+  if (fy) m ^= b.sym[(j < 2) ? 2 : 3];
+  long long s = (long long)sj * ix + si;
+  for (int n = 0; n < nvt; n++) {
+    double v = U[n * N + s];
+    U[n * N + k] = ((m >> n) & 1u) ? -v : v;
+  }
+}
+
+// End of cfl: combine the reduction slots into dt (Scheme_rot.f90:183-213).
+// mode 0: pip_cfl (optionally into the global dt); mode 2: the same at the
+// top of a fused loop (also sets active/last); mode 1: end of a fused step
+// (time += dt, stop_sim, next dt and cmax, activity flags).
+struct FinArgs {
+  DevScalars* sc;
+  int mode, into_global, flag_ir, flag_ir_type, flag_divb;
+  double safety, dxyz_min;
+};
+
+__global__ void k_finalize(FinArgs f) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevScalars* sc = f.sc;
+  if (f.mode == 1 && !sc->active) return;
+  if (f.mode == 1) sc->ion_pot_valid = 1;  // source_PIP ran in this step
+  double dttemp = dmin(10.0, as_f64(sc->red[RED_DT]));
+  dttemp = dmin(dttemp, dmin(f.safety, 0.3) / as_f64(sc->red[RED_RHOAC]));
+  if (f.flag_ir >= 1) {
+    dttemp = dmin(dttemp, dmin(f.safety, 0.1) / dmax(as_f64(sc->red[RED_GMAX]), 1.0e-5));
+    if (f.flag_ir_type == 0) {
+      double mx = sc->ion_pot_valid ? as_f64(sc->red[RED_IONPOT]) : 0.0;
+      dttemp = dmin(dttemp, f.safety / dmax(mx, 1.0e-5));
+    }
+  }
+  int nan = (int)(sc->red[RED_NAN] != 0ull);
+  sc->red[RED_DT] = as_u64(1.0e300);
+  sc->red[RED_RHOAC] = 0ull;
+  sc->red[RED_GMAX] = 0ull;
+  sc->red[RED_IONPOT] = 0ull;
+  sc->red[RED_NAN] = 0ull;
+  sc->dt_out = dttemp;
+  if (f.mode == 0 || f.mode == 2) {
+    if (f.into_global) sc->dt = dttemp;
+    if (f.flag_divb >= 1) sc->cmax = f.safety * f.dxyz_min / sc->dt;
+    sc->status = nan ? 2 : 0;
+    if (f.mode == 2) {  // top of the time loop: decide whether this step is the last
+      bool go = (sc->time < sc->tend) && (sc->nt < sc->nmax);
+      sc->active = go ? 1 : 0;
+      sc->last = ((sc->time + sc->dt >= sc->tend) || (sc->nt + 1 >= sc->nmax) || (sc->dt <= 1.0e-12)) ? 1 : 0;
+    }
+    return;
+  }
+  // fused step end: main.f90:58-62 then the next iteration's cfl(dt)
+  double dt_used = sc->dt;
+  sc->time = sc->time + dt_used;
+  sc->nt = sc->nt + 1;
+  int st = 0;
+  if (dt_used <= 1.0e-12) st |= 1;
+  if (nan) st |= 2;
+  sc->status = st;
+  bool more = (sc->time < sc->tend) && (sc->nt < sc->nmax) && (st == 0);
+  sc->active = more ? 1 : 0;
+  if (more) {
+    sc->dt = dttemp;
+    if (f.flag_divb >= 1) sc->cmax = f.safety * f.dxyz_min / sc->dt;
+    sc->last = ((sc->time + sc->dt >= sc->tend) || (sc->nt + 1 >= sc->nmax) || (sc->dt <= 1.0e-12)) ? 1 : 0;
+  }
+}
+
+}  // namespace pip
+
+namespace pip {
+// Small control kernel: set loop bookkeeping fields on the device.
+__global__ void k_set_loop(DevScalars* sc, double tend, long long nmax, int active, int ion_valid) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc->tend = tend;
+  sc->nmax = nmax;
+  if (active >= 0) sc->active = active;
+  if (ion_valid >= 0) sc->ion_pot_valid = ion_valid;
+}
+}  // namespace pip

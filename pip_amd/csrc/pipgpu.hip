@@ -1,0 +1,620 @@
+// pipgpu.hip -- C-ABI of libpipgpu.so (declared in include/pipgpu.h).
+//
+// Owns the device state of one domain per context, dispatches the stage,
+// rewrite/cfl, boundary and finalize kernels of pip_kernels.h on one HIP
+// stream (default priority) and mirrors the reference's procedures:
+// prologue, set_IR, cfl, run_solver, PIPbnd and the main.f90 time loop.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/pipgpu.h"
+#include "pip_kernels.h"
+
+using namespace pip;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(PIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+struct pip_ctx {
+  pip_cfg cfg;
+  int nv, nvt, nx, ny;
+  long long N;
+  double* U = nullptr;   // state (U_m planes then U_h planes)
+  double* U1 = nullptr;  // stage buffers
+  double* U2 = nullptr;
+  double* ac = nullptr;
+  double* gm_rec = nullptr;
+  double* gm_ion = nullptr;
+  double* arb_heat = nullptr;
+  double* ion_pot = nullptr;
+  DevScalars* sc = nullptr;
+  hipStream_t stream = nullptr;
+  int tx = 32, ty = 8;
+  Phys ph;
+  IRConst irc;
+  Bnd bnd;
+  double gmin_safety, dxyz_min;
+  bool arb_done = false;
+  bool top_pending = true;
+  int device = 0;
+};
+
+// --------------------------------------------------------------- helpers
+static int bnd_type(int f) {
+  if ((f >= 3 && f <= 7) || f == 11) return 2;
+  return f;
+}
+
+// set_sym_mhd / set_sym_hd (Boundary_rot.f90:239-320) as sign bit masks.
+static void build_bnd(pip_ctx* c) {
+  const pip_cfg& g = c->cfg;
+  for (int s = 0; s < 4; s++) {
+    c->bnd.type[s] = 0;
+    c->bnd.sym[s] = 0u;
+  }
+  for (int i = 1; i <= 2 * g.ndim; i++) {
+    int dir = (i + 1) / 2, f = g.flag_bnd[i - 1];
+    int sm[9], sh[5];
+    for (int n = 0; n < 9; n++) sm[n] = 1;
+    for (int n = 0; n < 5; n++) sh[n] = 1;
+    if (f == 2) { sm[dir] = -1; sm[5 + dir - 1] = -1; }
+    else if (f == 3) { sm[dir] = -1; sm[6 + dir % 3 - 1] = -1; sm[2 + (dir + 1) % 3 - 1] = -1; }
+    else if (f == 4) { sm[dir] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; sm[2 + dir % 3 - 1] = -1; }
+    else if (f == 5) { sm[dir] = -1; sm[5 + dir - 1] = -1; sm[2 + (dir + 1) % 3 - 1] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; }
+    else if (f == 6) { sm[dir] = -1; sm[6 + dir % 3 - 1] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; }
+    else if (f == 7) { sm[1] = -1; sm[2] = -1; sm[3] = -1; }
+    if (g.flag_divb == 1) {
+      if (f == 2 || f == 5) sm[8] = 1;
+      else if (f == 3 || f == 4 || f == 6 || f == 7) sm[8] = -1;
+    }
+    if (f == 2) sh[dir] = -1;
+    else if (f == 3) { sh[dir] = -1; sh[2 + (dir + 1) % 3 - 1] = -1; }
+    else if (f == 4) { sh[dir] = -1; sh[2 + dir % 3 - 1] = -1; }
+    else if (f == 5) { sh[dir] = -1; sh[2 + (dir + 1) % 3 - 1] = -1; }
+    else if (f == 6) sh[dir] = -1;
+    unsigned m = 0u;
+    for (int n = 0; n < c->nv; n++)
+      if (sm[n] < 0) m |= 1u << n;
+    for (int n = 0; n < 5; n++)
+      if (sh[n] < 0) m |= 1u << (c->nv + n);
+    c->bnd.type[i - 1] = bnd_type(f);
+    c->bnd.sym[i - 1] = m;
+  }
+}
+
+static DevScalars host_scalars_init() {
+  DevScalars s;
+  memset(&s, 0, sizeof(s));
+  s.dt = 0.0;
+  s.time = 0.0;
+  s.cmax = 0.0;
+  s.tend = 1.0e300;
+  s.red[RED_DT] = as_u64(1.0e300);
+  s.nmax = (long long)1 << 62;
+  s.active = 1;
+  return s;
+}
+
+static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const double* U0, double* Uout,
+                            int epi, double* ion_out) {
+  StageArgs a;
+  a.Uin = Uin;
+  a.U0 = U0;
+  a.Uout = Uout;
+  a.ac = c->cfg.ac_field ? c->ac : nullptr;
+  a.arb_heat = c->arb_heat;
+  a.ion_pot_out = ion_out;
+  a.sc = c->sc;
+  a.ix = c->cfg.ix;
+  a.jx = c->cfg.jx;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.N = c->N;
+  a.istep = istep;
+  a.t_order = c->cfg.t_order;
+  a.epi = epi;
+  a.col = c->cfg.col;
+  a.gmin_safety = c->gmin_safety;
+  a.dxyz_min = c->dxyz_min;
+  a.dx = c->cfg.dx;
+  a.dy = c->cfg.dy;
+  a.db_clean = c->cfg.db_clean;
+  a.ph = c->ph;
+  a.irc = c->irc;
+  a.bnd = c->bnd;
+  return a;
+}
+
+template <int NDIM, int NV, bool IR, int TX, int TY>
+static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
+  constexpr int NVT = NV + 5;
+  constexpr int HX = TX + 4;
+  constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
+  constexpr int NFX = (TX + 1) * TY;
+  constexpr int NFY = (NDIM == 2) ? TX * (TY + 1) : 0;
+  constexpr int NF = (NFX > NFY) ? NFX : NFY;
+  constexpr size_t lds = sizeof(double) * (size_t)(NVT * HY * HX + NVT * NF);
+  static bool attr_done = false;
+  auto kern = k_stage<NDIM, NV, IR, TX, TY>;
+  if (!attr_done) {
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_done = true;
+  }
+  int ntx = (c->nx + TX - 1) / TX;
+  int nty = (NDIM == 2) ? (c->ny + TY - 1) / TY : 1;
+  hipLaunchKernelGGL(kern, dim3(ntx * nty), dim3(TX * TY), lds, c->stream, a);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+template <int NDIM, int NV, bool IR>
+static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
+  if constexpr (NDIM == 1) {
+    return launch_stage_t<1, NV, IR, 256, 1>(c, a);
+  } else {
+    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, 32, 8>(c, a);
+    if (c->tx == 16 && c->ty == 16) return launch_stage_t<2, NV, IR, 16, 16>(c, a);
+    if (c->tx == 64 && c->ty == 4) return launch_stage_t<2, NV, IR, 64, 4>(c, a);
+    if (c->tx == 32 && c->ty == 4) return launch_stage_t<2, NV, IR, 32, 4>(c, a);
+    if (c->tx == 16 && c->ty == 8) return launch_stage_t<2, NV, IR, 16, 8>(c, a);
+    return fail(PIP_ERR_ARG, "unsupported tile shape");
+  }
+}
+
+static int launch_stage(pip_ctx* c, const StageArgs& a) {
+  const bool ir = c->cfg.flag_ir >= 2;
+  if (c->cfg.ndim == 1) {
+    if (c->nv != 8) return fail(PIP_ERR_UNSUPPORTED, "1-D runs have nvar_m = 8");
+    return ir ? launch_stage_tile<1, 8, true>(c, a) : launch_stage_tile<1, 8, false>(c, a);
+  }
+  if (c->nv == 9)
+    return ir ? launch_stage_tile<2, 9, true>(c, a) : launch_stage_tile<2, 9, false>(c, a);
+  return ir ? launch_stage_tile<2, 8, true>(c, a) : launch_stage_tile<2, 8, false>(c, a);
+}
+
+static RoundArgs round_args(pip_ctx* c, int do_ir, int init_arb, int do_cfl, int read_rates,
+                            const double* ion_pot) {
+  RoundArgs r;
+  r.U = c->U;
+  r.ac = c->cfg.ac_field ? c->ac : nullptr;
+  r.gm_rec = c->gm_rec;
+  r.gm_ion = c->gm_ion;
+  r.arb_heat = c->arb_heat;
+  r.ion_pot = ion_pot;
+  r.sc = c->sc;
+  r.ix = c->cfg.ix;
+  r.jx = c->cfg.jx;
+  r.nx = c->nx;
+  r.ny = c->ny;
+  r.N = c->N;
+  r.do_ir = do_ir;
+  r.init_arb = init_arb;
+  r.do_cfl = do_cfl;
+  r.read_rates = read_rates;
+  r.col = c->cfg.col;
+  r.gmin_safety = c->gmin_safety;
+  r.ph = c->ph;
+  r.irc = c->irc;
+  return r;
+}
+
+static int launch_round(pip_ctx* c, const RoundArgs& r) {
+  dim3 grid((unsigned)((c->N + 255) / 256));
+  const bool ir = c->cfg.flag_ir >= 1;
+  if (c->nv == 9) {
+    if (ir) hipLaunchKernelGGL((k_round<9, true>), grid, dim3(256), 0, c->stream, r);
+    else hipLaunchKernelGGL((k_round<9, false>), grid, dim3(256), 0, c->stream, r);
+  } else {
+    if (ir) hipLaunchKernelGGL((k_round<8, true>), grid, dim3(256), 0, c->stream, r);
+    else hipLaunchKernelGGL((k_round<8, false>), grid, dim3(256), 0, c->stream, r);
+  }
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+static int launch_finalize(pip_ctx* c, int mode, int into_global) {
+  FinArgs f;
+  f.sc = c->sc;
+  f.mode = mode;
+  f.into_global = into_global;
+  f.flag_ir = c->cfg.flag_ir;
+  f.flag_ir_type = c->cfg.flag_ir_type;
+  f.flag_divb = c->cfg.flag_divb;
+  f.safety = c->cfg.safety;
+  f.dxyz_min = c->dxyz_min;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, c->stream, f);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+static int launch_bnd(pip_ctx* c, double* U) {
+  dim3 grid((unsigned)((c->N + 255) / 256));
+  hipLaunchKernelGGL(k_bnd, grid, dim3(256), 0, c->stream, U, c->nvt, c->cfg.ix, c->cfg.jx, c->N, c->bnd,
+                     c->cfg.ndim);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+static int set_loop(pip_ctx* c, double tend, long long nmax, int active, int ion_valid) {
+  hipLaunchKernelGGL(k_set_loop, dim3(1), dim3(64), 0, c->stream, c->sc, tend, nmax, active, ion_valid);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+#define TRY(x)            \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_ != PIP_OK) return rc_; \
+  } while (0)
+
+// ------------------------------------------------------------------ C-ABI
+extern "C" {
+
+const char* pip_last_error(void) { return g_err.c_str(); }
+const char* pip_version(void) { return "pipgpu 0.1 (gfx950, fp64)"; }
+
+int pip_create(const pip_cfg* cfg, pip_ctx** out) {
+  if (!cfg || !out) return fail(PIP_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->ndim != 1 && cfg->ndim != 2) return fail(PIP_ERR_ARG, "ndim must be 1 or 2");
+  if (cfg->ix < 8) return fail(PIP_ERR_ARG, "ix must be >= 8 (nx >= 4)");
+  if (cfg->ndim == 1 && cfg->jx != 1) return fail(PIP_ERR_ARG, "jx must be 1 in 1-D");
+  if (cfg->ndim == 2 && cfg->jx < 8) return fail(PIP_ERR_ARG, "jx must be >= 8 (ny >= 4)");
+  int want_nv = 8 + ((cfg->ndim == 2 && cfg->flag_divb == 1) ? 1 : 0);
+  if (cfg->nvar_m != want_nv) return fail(PIP_ERR_ARG, "nvar_m inconsistent with ndim/flag_divb");
+  if (cfg->t_order != 2 && cfg->t_order != 3) return fail(PIP_ERR_UNSUPPORTED, "t_order must be 2 or 3");
+  if (cfg->flag_ir != 0 && cfg->flag_ir != 3)
+    return fail(PIP_ERR_UNSUPPORTED, "flag_IR must be 0 or 3 (set_IR case 3)");
+  if (cfg->flag_ir_type < 0 || cfg->flag_ir_type > 2) return fail(PIP_ERR_ARG, "flag_IR_type in 0..2");
+  for (int s = 0; s < 2 * cfg->ndim; s++) {
+    int t = bnd_type(cfg->flag_bnd[s]);
+    if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
+  }
+  if (!(cfg->dx > 0) || (cfg->ndim == 2 && !(cfg->dy > 0))) return fail(PIP_ERR_ARG, "dx, dy must be > 0");
+  pip_ctx* c = new pip_ctx();
+  c->cfg = *cfg;
+  c->nv = cfg->nvar_m;
+  c->nvt = c->nv + 5;
+  c->nx = cfg->ix - 4;
+  c->ny = (cfg->ndim == 2) ? cfg->jx - 4 : 1;
+  c->N = (long long)cfg->ix * cfg->jx;
+  c->ph.gm = cfg->gm;
+  c->ph.ro_lim = cfg->ro_lim;
+  c->ph.pr_lim = cfg->pr_lim;
+  c->ph.safety = cfg->safety;
+  c->ph.T0 = cfg->T0;
+  c->ph.n0 = cfg->n0;
+  c->ph.beta = cfg->beta;
+  c->ph.t_ir = cfg->t_ir;
+  c->ph.flag_col = cfg->flag_col;
+  c->ph.flag_ir = cfg->flag_ir;
+  c->ph.flag_ir_type = cfg->flag_ir_type;
+  c->irc = ir_constants(c->ph);
+  double dy = (cfg->ndim >= 2) ? cfg->dy : 100.0, dz = 100.0;  // Model_rot.f90:424-429
+  c->dxyz_min = dmin(dmin(cfg->dx, dy), dz);
+  c->gmin_safety = cfg->safety * c->dxyz_min;
+  build_bnd(c);
+  hipError_t e;
+  size_t bytes = sizeof(double) * (size_t)c->N;
+  auto cleanup = [&](const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    pip_destroy(c);
+    return fail(PIP_ERR_HIP, m);
+  };
+  if ((e = hipGetDevice(&c->device)) != hipSuccess) return cleanup("hipGetDevice");
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return cleanup("hipStreamCreate");
+  if ((e = hipMalloc(&c->U, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U");
+  if ((e = hipMalloc(&c->U1, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U1");
+  if (cfg->t_order >= 3 && (e = hipMalloc(&c->U2, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U2");
+  if (cfg->ac_field && (e = hipMalloc(&c->ac, bytes)) != hipSuccess) return cleanup("hipMalloc ac");
+  if (cfg->flag_ir) {
+    if ((e = hipMalloc(&c->gm_rec, bytes)) != hipSuccess) return cleanup("hipMalloc gm_rec");
+    if ((e = hipMalloc(&c->gm_ion, bytes)) != hipSuccess) return cleanup("hipMalloc gm_ion");
+    if ((e = hipMalloc(&c->arb_heat, bytes)) != hipSuccess) return cleanup("hipMalloc arb_heat");
+    if ((e = hipMalloc(&c->ion_pot, bytes)) != hipSuccess) return cleanup("hipMalloc ion_pot");
+    if ((e = hipMemset(c->arb_heat, 0, bytes)) != hipSuccess) return cleanup("hipMemset");
+    if ((e = hipMemset(c->ion_pot, 0, bytes)) != hipSuccess) return cleanup("hipMemset");
+  }
+  if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess) return cleanup("hipMalloc scalars");
+  DevScalars hs = host_scalars_init();
+  if ((e = hipMemcpy(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) return cleanup("hipMemcpy");
+  if ((e = hipMemset(c->U, 0, bytes * c->nvt)) != hipSuccess) return cleanup("hipMemset");
+  if ((e = hipMemset(c->U1, 0, bytes * c->nvt)) != hipSuccess) return cleanup("hipMemset");
+  *out = c;
+  return PIP_OK;
+}
+
+int pip_destroy(pip_ctx* c) {
+  if (!c) return PIP_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->U);
+  (void)hipFree(c->U1);
+  (void)hipFree(c->U2);
+  (void)hipFree(c->ac);
+  (void)hipFree(c->gm_rec);
+  (void)hipFree(c->gm_ion);
+  (void)hipFree(c->arb_heat);
+  (void)hipFree(c->ion_pot);
+  (void)hipFree(c->sc);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PIP_OK;
+}
+
+int pip_upload(pip_ctx* c, const double* U_h, const double* U_m, const double* ac) {
+  if (!c || !U_h || !U_m) return fail(PIP_ERR_ARG, "null argument");
+  size_t pb = sizeof(double) * (size_t)c->N;
+  HIPCHK(hipMemcpyAsync(c->U, U_m, pb * c->nv, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->U + (size_t)c->nv * c->N, U_h, pb * 5, hipMemcpyHostToDevice, c->stream));
+  if (c->cfg.ac_field) {
+    if (!ac) return fail(PIP_ERR_ARG, "ac_field=1 needs the ac array");
+    HIPCHK(hipMemcpyAsync(c->ac, ac, pb, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->top_pending = true;
+  return PIP_OK;
+}
+
+int pip_download(pip_ctx* c, double* U_h, double* U_m) {
+  if (!c || !U_h || !U_m) return fail(PIP_ERR_ARG, "null argument");
+  size_t pb = sizeof(double) * (size_t)c->N;
+  HIPCHK(hipMemcpyAsync(U_m, c->U, pb * c->nv, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(U_h, c->U + (size_t)c->nv * c->N, pb * 5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PIP_OK;
+}
+
+int pip_bnd(pip_ctx* c) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  TRY(launch_bnd(c, c->U));
+  return PIP_OK;
+}
+
+static int read_dt_out(pip_ctx* c, double* dt) {
+  DevScalars hs;
+  HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dt) *dt = hs.dt_out;
+  return PIP_OK;
+}
+
+int pip_set_ir(pip_ctx* c) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (c->cfg.flag_ir == 0) return PIP_OK;
+  TRY(launch_round(c, round_args(c, 1, c->arb_done ? 0 : 1, 0, 0, nullptr)));
+  c->arb_done = true;
+  return PIP_OK;
+}
+
+int pip_cfl(pip_ctx* c, double* dt, int into_global) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  TRY(launch_round(c, round_args(c, 0, 0, 1, c->cfg.flag_ir ? 1 : 0, c->ion_pot)));
+  TRY(launch_finalize(c, 0, into_global));
+  return read_dt_out(c, dt);
+}
+
+int pip_prologue(pip_ctx* c, double* dt) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  DevScalars hs = host_scalars_init();
+  HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
+  c->arb_done = false;
+  TRY(launch_bnd(c, c->U));        // boundary map of the initial state
+  TRY(pip_set_ir(c));              // set_coefficients(U_h,U_m,0) (Initial_rot.f90:140)
+  TRY(pip_cfl(c, dt, 1));          // cfl(U_h,U_m,dt) (Initial_rot.f90:152)
+  c->top_pending = true;
+  return PIP_OK;
+}
+
+// Stage s reads U (s=1), U1 (s=2) or U2 (s=3) and writes U1, U2 or, for the
+// last stage, U in place (each cell reads U0 only at itself).
+static int run_stages(pip_ctx* c, int epi_last, double* ion_out) {
+  const int T = c->cfg.t_order;
+  for (int s = 1; s <= T; s++) {
+    const double* Uin = (s == 1) ? c->U : ((s == 2) ? c->U1 : c->U2);
+    double* Uout = (s == T) ? c->U : ((s == 1) ? c->U1 : c->U2);
+    int epi = (s == T) ? epi_last : 0;
+    StageArgs a = stage_args(c, s, Uin, c->U, Uout, epi, ion_out);
+    TRY(launch_stage(c, a));
+  }
+  return PIP_OK;
+}
+
+int pip_run_solver(pip_ctx* c) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  TRY(set_loop(c, 1.0e300, (long long)1 << 62, 1, -1));
+  TRY(run_stages(c, 0, c->ion_pot));
+  TRY(set_loop(c, 1.0e300, (long long)1 << 62, -1, 1));  // ion_pot now defined
+  return PIP_OK;
+}
+
+static int loop_top(pip_ctx* c, double tend, long long nmax) {
+  // first pass of main.f90:32-36 (set_IR, cfl(dt)) when no fused step has
+  // already applied it as its look-ahead round
+  TRY(set_loop(c, tend, nmax, 1, -1));
+  if (c->cfg.flag_ir) TRY(launch_round(c, round_args(c, 1, c->arb_done ? 0 : 1, 0, 0, nullptr)));
+  c->arb_done = true;
+  TRY(launch_round(c, round_args(c, 0, 0, 1, c->cfg.flag_ir ? 1 : 0, c->ion_pot)));
+  TRY(launch_finalize(c, 2, 1));
+  c->top_pending = false;
+  return PIP_OK;
+}
+
+static int enqueue_fused(pip_ctx* c, int n) {
+  for (int k = 0; k < n; k++) {
+    TRY(run_stages(c, 2, nullptr));
+    TRY(launch_finalize(c, 1, 0));
+  }
+  return PIP_OK;
+}
+
+int pip_advance(pip_ctx* c, int max_steps, double tend, int* steps_done, double* time) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (max_steps < 0) return fail(PIP_ERR_ARG, "max_steps < 0");
+  DevScalars hs;
+  HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  long long nt0 = hs.nt;
+  long long nmax = nt0 + max_steps;
+  TRY(loop_top(c, tend, nmax));
+  const int chunk = 64;
+  int left = max_steps;
+  while (left > 0) {
+    int n = left < chunk ? left : chunk;
+    TRY(enqueue_fused(c, n));
+    left -= n;
+    HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!hs.active) break;
+  }
+  c->top_pending = true;
+  if (steps_done) *steps_done = (int)(hs.nt - nt0);
+  if (time) *time = hs.time;
+  if (hs.status) g_err = (hs.status & 2) ? "NaN in U_m" : "dt too small";
+  return PIP_OK;
+}
+
+int pip_enqueue_steps(pip_ctx* c, int n, double tend) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (c->top_pending) {
+    DevScalars hs;
+    HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TRY(loop_top(c, tend, (long long)1 << 62));
+  }
+  return enqueue_fused(c, n);
+}
+
+int pip_sync(pip_ctx* c) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PIP_OK;
+}
+
+int pip_get_scalars(pip_ctx* c, double* dt, double* time, double* cmax, long* nt) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  DevScalars hs;
+  HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dt) *dt = hs.dt;
+  if (time) *time = hs.time;
+  if (cmax) *cmax = hs.cmax;
+  if (nt) *nt = (long)hs.nt;
+  return PIP_OK;
+}
+
+int pip_set_time(pip_ctx* c, double time, double dt) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  DevScalars hs;
+  HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hs.time = time;
+  hs.dt = dt;
+  if (c->cfg.flag_divb >= 1) hs.cmax = c->cfg.safety * c->dxyz_min / dt;
+  HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PIP_OK;
+}
+
+int pip_status(pip_ctx* c, int* flags) {
+  if (!c || !flags) return fail(PIP_ERR_ARG, "null argument");
+  DevScalars hs;
+  HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *flags = hs.status;
+  return PIP_OK;
+}
+
+int pip_get_rates(pip_ctx* c, double* gm_rec, double* gm_ion, double* arb_heat) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (!c->cfg.flag_ir) return fail(PIP_ERR_STATE, "flag_IR = 0: no rate fields");
+  size_t pb = sizeof(double) * (size_t)c->N;
+  if (gm_rec) HIPCHK(hipMemcpyAsync(gm_rec, c->gm_rec, pb, hipMemcpyDeviceToHost, c->stream));
+  if (gm_ion) HIPCHK(hipMemcpyAsync(gm_ion, c->gm_ion, pb, hipMemcpyDeviceToHost, c->stream));
+  if (arb_heat) HIPCHK(hipMemcpyAsync(arb_heat, c->arb_heat, pb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PIP_OK;
+}
+
+int pip_device_state(pip_ctx* c, double** U, double** ac) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (U) *U = c->U;
+  if (ac) *ac = c->ac;
+  return PIP_OK;
+}
+
+int pip_get_stream(pip_ctx* c, void** stream) {
+  if (!c || !stream) return fail(PIP_ERR_ARG, "null argument");
+  *stream = (void*)c->stream;
+  return PIP_OK;
+}
+
+int pip_set_tile(pip_ctx* c, int tx, int ty) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  const int ok[][2] = {{32, 8}, {16, 16}, {64, 4}, {32, 4}, {16, 8}};
+  for (auto& t : ok)
+    if (t[0] == tx && t[1] == ty) {
+      c->tx = tx;
+      c->ty = ty;
+      return PIP_OK;
+    }
+  return fail(PIP_ERR_ARG, "unsupported tile shape");
+}
+
+int pip_time_steps(pip_ctx* c, int nsteps, double* ms_total, double* ms_stage1, double* ms_stage2) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (c->cfg.t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "timing helper covers t_order = 2");
+  if (c->top_pending) TRY(loop_top(c, 1.0e300, (long long)1 << 62));
+  const int NE = 3 * nsteps + 1;
+  hipEvent_t* ev = new hipEvent_t[NE];
+  for (int k = 0; k < NE; k++) HIPCHK(hipEventCreate(&ev[k]));
+  HIPCHK(hipEventRecord(ev[0], c->stream));
+  int rc = PIP_OK;
+  for (int k = 0; k < nsteps && rc == PIP_OK; k++) {
+    StageArgs a1 = stage_args(c, 1, c->U, c->U, c->U1, 0, nullptr);
+    rc = launch_stage(c, a1);
+    if (rc != PIP_OK) break;
+    HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
+    StageArgs a2 = stage_args(c, 2, c->U1, c->U, c->U, 2, nullptr);
+    rc = launch_stage(c, a2);
+    if (rc != PIP_OK) break;
+    HIPCHK(hipEventRecord(ev[3 * k + 2], c->stream));
+    rc = launch_finalize(c, 1, 0);
+    if (rc != PIP_OK) break;
+    HIPCHK(hipEventRecord(ev[3 * k + 3], c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double t1 = 0, t2 = 0, tt = 0;
+  float ms;
+  for (int k = 0; k < nsteps && rc == PIP_OK; k++) {
+    HIPCHK(hipEventElapsedTime(&ms, ev[3 * k], ev[3 * k + 1]));
+    t1 += ms;
+    HIPCHK(hipEventElapsedTime(&ms, ev[3 * k + 1], ev[3 * k + 2]));
+    t2 += ms;
+  }
+  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[NE - 1]));
+  tt = ms;
+  for (int k = 0; k < NE; k++) (void)hipEventDestroy(ev[k]);
+  delete[] ev;
+  if (ms_total) *ms_total = tt;
+  if (ms_stage1) *ms_stage1 = t1;
+  if (ms_stage2) *ms_stage2 = t2;
+  c->top_pending = true;
+  return rc;
+}
+
+}  // extern "C"

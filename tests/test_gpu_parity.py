@@ -1,0 +1,200 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Both paths evaluate the reference arithmetic operation by operation with no
+FMA contraction and with the same fixed exp/log/pow sequences, so they must
+agree bit for bit (shocks and ionisation/recombination included), which is
+stricter than the 1e-12 relative L1 bar.  Full-size runs are also checked
+through conservation to round-off.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from pip_amd import problems
+from pip_amd.solver import PIPSolver
+from tests.util import interior, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(cfg, uh, um, ac=None, nsteps=None, tend=None, tile=None):
+    o = Oracle(cfg, uh, um, ac)
+    o.run(nsteps=nsteps, tend=tend)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
+    n, t = s.advance(nsteps=nsteps, tend=tend)
+    return o, s, n, t
+
+
+def compare(o, s, cfg, tol):
+    oh, om = o.state()
+    gh, gm = s.state()
+    if tol == 0.0:
+        assert np.array_equal(interior(gm, cfg), interior(om, cfg))
+        assert np.array_equal(interior(gh, cfg), interior(oh, cfg))
+        say("bitwise equal")
+        return 0.0, 0.0
+    eh = rel_l1(gh, oh, cfg)
+    em = rel_l1(gm, om, cfg)
+    say("rel L1 U_m", np.array2string(em, precision=2), "U_h", np.array2string(eh, precision=2))
+    assert np.all(np.isfinite(gm)) and np.all(np.isfinite(gh))
+    assert em.max() <= tol, em
+    assert eh.max() <= tol, eh
+    return em, eh
+
+
+def test_brio_wu_1d_to_tend():
+    """Config 1: 1-D two-fluid Brio-Wu, nx=1024, t_end=0.1, outflow."""
+    say("case brio_wu nx=1024 tend=0.1")
+    cfg, uh, um, ac = problems.brio_wu(1024)
+    o, s, n, t = run_pair(cfg, uh, um, ac, tend=0.1)
+    so = o.scalars()
+    assert n == so["nt"]
+    assert t == so["time"]
+    compare(o, s, cfg, 0.0)
+    # shock positions: sign changes of By and the density jumps coincide
+    oh, om = o.state()
+    gh, gm = s.state()
+    for U, G in ((om, gm), (oh, gh)):
+        jo = np.argmax(np.abs(np.diff(U[0, 0, 2:-2])))
+        jg = np.argmax(np.abs(np.diff(G[0, 0, 2:-2])))
+        assert jo == jg
+
+
+def test_orszag_tang_small_100_steps():
+    say("case orszag_tang n=64 steps=100")
+    cfg, uh, um, ac = problems.orszag_tang(64)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=100)
+    assert n == 100
+    compare(o, s, cfg, 0.0)
+    assert s.scalars()["dt"] == o.scalars()["dt"]
+
+
+def test_kelvin_helmholtz_ir_small():
+    say("case kelvin_helmholtz n=64 steps=100 (IR case 3, mirror y)")
+    cfg, uh, um, ac = problems.kelvin_helmholtz(64)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=100)
+    assert n == 100
+    compare(o, s, cfg, 0.0)
+
+
+def test_alfven_per_cell_alpha():
+    say("case alfven n=64 steps=100 (per-cell alpha_c)")
+    cfg, uh, um, ac = problems.alfven_damping(64)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=100)
+    compare(o, s, cfg, 0.0)
+
+
+def test_ssprk3():
+    say("case orszag_tang n=32 t_order=3 steps=40")
+    cfg, uh, um, ac = problems.orszag_tang(32)
+    cfg = cfg.replace(t_order=3)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=40)
+    compare(o, s, cfg, 0.0)
+
+
+def test_pncoupling_uniform():
+    say("case pncoupling nx=16 steps=50")
+    cfg, uh, um, ac = problems.pn_coupling(nx=16, xi_n=0.7, col=10.0, v0=0.2)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=50)
+    compare(o, s, cfg, 0.0)
+
+
+def test_unfused_procedures_match_fused_loop():
+    """main.f90's loop through pip_set_ir / pip_cfl / pip_run_solver equals
+    the fused device loop."""
+    say("case kelvin_helmholtz n=48 unfused vs fused, 25 steps")
+    cfg, uh, um, ac = problems.kelvin_helmholtz(48)
+    a = PIPSolver(cfg, uh, um, ac)
+    for _ in range(25):
+        a.step()
+    b = PIPSolver(cfg, uh, um, ac)
+    b.advance(nsteps=25)
+    ah, am = a.state()
+    bh, bm = b.state()
+    assert np.array_equal(interior(am, cfg), interior(bm, cfg))
+    assert np.array_equal(interior(ah, cfg), interior(bh, cfg))
+    assert a.time_host == b.scalars()["time"]
+
+
+@pytest.mark.parametrize("tile", [(32, 8), (16, 16), (64, 4), (32, 4), (16, 8)])
+def test_tile_shapes_bitwise_equal(tile):
+    say("case orszag_tang n=72 tile", tile)
+    cfg, uh, um, ac = problems.orszag_tang(72)
+    ref = PIPSolver(cfg, uh, um, ac)
+    ref.advance(nsteps=8)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
+    s.advance(nsteps=8)
+    rh, rm = ref.state()
+    gh, gm = s.state()
+    assert np.array_equal(interior(rm, cfg), interior(gm, cfg))
+    assert np.array_equal(interior(rh, cfg), interior(gh, cfg))
+
+
+def test_full_size_orszag_tang_two_steps_vs_oracle():
+    """Config 2 at its real size (1024^2): two steps against the oracle."""
+    say("case orszag_tang n=1024 steps=2 vs oracle")
+    cfg, uh, um, ac = problems.orszag_tang(1024)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=2)
+    compare(o, s, cfg, 0.0)
+
+
+def test_full_size_conservation_orszag_tang():
+    """Size-independent property at 1024^2: periodic box conserves each
+    fluid's mass and the total momentum and energy to round-off."""
+    say("case orszag_tang n=1024 steps=50 conservation")
+    cfg, uh, um, ac = problems.orszag_tang(1024)
+    s = PIPSolver(cfg, uh, um, ac)
+    h0, m0 = s.state()
+    s.advance(nsteps=50)
+    h1, m1 = s.state()
+
+    def tot(h, m):
+        h, m = interior(h, cfg), interior(m, cfg)
+        return np.array([h[0].sum(), m[0].sum(), (h[1] + m[1]).sum(), (h[2] + m[2]).sum(),
+                         (h[4] + m[4]).sum()])
+    t0, t1 = tot(h0, m0), tot(h1, m1)
+    scale = np.array([t0[0], t0[1], np.abs(h0[1]).sum(), np.abs(h0[2]).sum(), t0[4]])
+    err = np.abs(t1 - t0) / scale
+    say("conservation errors", err)
+    assert np.all(err < 1e-12)
+
+
+def test_full_size_kh_mass_conservation():
+    """Config 3 at 2048^2, 20 steps: total (neutral + ion) mass is conserved
+    exactly up to round-off (ionisation/recombination exchange mass; walls
+    have zero mass flux)."""
+    say("case kelvin_helmholtz n=2048 steps=20 mass")
+    cfg, uh, um, ac = problems.kelvin_helmholtz(2048)
+    s = PIPSolver(cfg, uh, um, ac)
+    h0, m0 = s.state()
+    s.advance(nsteps=20)
+    h1, m1 = s.state()
+    M0 = interior(h0, cfg)[0].sum() + interior(m0, cfg)[0].sum()
+    M1 = interior(h1, cfg)[0].sum() + interior(m1, cfg)[0].sum()
+    say("mass", M0, M1)
+    assert abs(M1 - M0) / M0 < 1e-12
+    assert s.status() == 0
+
+
+def test_stop_sim_nan_and_small_dt():
+    """stop_sim conditions forced from the host (IO_rot.f90:625-674)."""
+    say("case stop_sim")
+    cfg, uh, um, ac = problems.brio_wu(64)
+    um2 = um.copy()
+    # B_y: a NaN in the energy would be replaced by the pressure floor of
+    # cq2pv (max(NaN, pr_lim) = pr_lim in both implementations)
+    um2[6, 0, 30] = np.nan
+    s = PIPSolver(cfg, uh, um2, ac)
+    n, _ = s.advance(nsteps=10)
+    assert n == 1 and (s.status() & 2)
+    cfg2, uh, um, ac = problems.brio_wu(64, col=1.0e13)
+    s = PIPSolver(cfg2, uh, um, ac)
+    n, _ = s.advance(nsteps=10)
+    assert n == 1 and (s.status() & 1)

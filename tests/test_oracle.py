@@ -411,3 +411,34 @@ def test_dedner_psi_damping_factor():
     f1, f2 = math.exp(-s * d), math.exp(-0.5 * s * d)
     g = 0.5 * (1 + f1) * f2
     assert m[8, 5, 5] == pytest.approx(0.7 * g ** 3, rel=1e-13)
+
+
+def test_portable_exp_log_pow_close_to_libm():
+    """The oracle's fixed exp/log/pow sequences (used identically on the GPU)
+    stay within a few ulp of the C library over the ranges the path uses."""
+    xs = np.concatenate([np.linspace(-745.0, 709.0, 4001), RNG.uniform(-60, 5, 4000),
+                         [-1e-300, 0.0, 1e-300, 0.5, 1.0, -0.1, -0.21]])
+    worst = 0.0
+    for x in xs:
+        ref = math.exp(x)
+        got = orc.lib().orc_exp(float(x))
+        if ref == 0.0 or ref < 1e-300:
+            assert abs(got - ref) <= 1e-300
+            continue
+        worst = max(worst, abs(got - ref) / math.ulp(ref))
+    assert worst <= 3.0, worst
+    worst = 0.0
+    for x in np.concatenate([np.logspace(-300, 300, 4001), RNG.uniform(0.5, 2.0, 4000)]):
+        ref = math.log(x)
+        got = orc.lib().orc_log(float(x))
+        if ref == 0.0:
+            assert got == 0.0
+            continue
+        worst = max(worst, abs(got - ref) / math.ulp(ref))
+    assert worst <= 3.0, worst
+    worst = 0.0
+    for x in np.concatenate([np.logspace(-3, 4, 3001), RNG.uniform(1.0, 200.0, 3000)]):
+        ref = x ** 0.39
+        got = orc.lib().orc_pow(float(x), 0.39)
+        worst = max(worst, abs(got - ref) / math.ulp(ref))
+    assert worst <= 6.0, worst
