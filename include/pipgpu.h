@@ -104,6 +104,33 @@ int pip_set_tile(pip_ctx *ctx, int tx, int ty);
  * measured with hipEvents on the context's stream. */
 int pip_time_steps(pip_ctx *ctx, int nsteps, double *ms_total, double *ms_stage1, double *ms_stage2);
 
+/* ---- batched ensemble (no counterpart in the reference: PIP runs one
+ * domain per process).  Independent 1-D two-fluid runs of different sizes,
+ * each with its own dt/time/step count, integrated in ONE persistent launch
+ * (optionally replayed from a hipGraph); every member equals its plain
+ * pip_advance run bit for bit. */
+typedef struct pip_ens_member {
+    int nx;            /* interior cells (ix = nx + 4)                     */
+    double dx;         /* cell width                                      */
+    double col;        /* alpha_c (flag_col = 1)                           */
+    double tend;       /* end time of this run (main.f90:31)               */
+    int max_steps;     /* step cap                                        */
+    double cost;       /* expected work, for longest-first scheduling     */
+} pip_ens_member;
+typedef struct pip_ens pip_ens;
+/* base supplies gm, safety, limits, flag_bnd, db_clean; ndim must be 1. */
+int pip_ens_create(const pip_cfg *base, const pip_ens_member *members, int n, pip_ens **out);
+int pip_ens_upload(pip_ens *e, int m, const double *U_h, const double *U_m);
+int pip_ens_download(pip_ens *e, int m, double *U_h, double *U_m);
+/* workgroup width (cells per tile = threads): 64, 128, 256 or 512 */
+int pip_ens_set_tile(pip_ens *e, int tx);
+/* restore every member to its uploaded state (device copy, not timed) */
+int pip_ens_reset(pip_ens *e);
+/* run all members to their end; ms (may be NULL) = device time of the launch */
+int pip_ens_run(pip_ens *e, int use_graph, double *ms);
+int pip_ens_result(pip_ens *e, int m, double *time, long *nt, int *status);
+int pip_ens_destroy(pip_ens *e);
+
 const char *pip_last_error(void);
 const char *pip_version(void);
 

@@ -44,6 +44,12 @@ def cfg_to_c(cfg) -> PipCfg:
     return c
 
 
+class PipEnsMember(ctypes.Structure):
+    """pip_ens_member of include/pipgpu.h."""
+    _fields_ = [("nx", ctypes.c_int), ("dx", ctypes.c_double), ("col", ctypes.c_double),
+                ("tend", ctypes.c_double), ("max_steps", ctypes.c_int), ("cost", ctypes.c_double)]
+
+
 class PipError(RuntimeError):
     pass
 
@@ -79,6 +85,14 @@ _SIGS = {
     "pip_get_stream": [_V, _PP],
     "pip_set_tile": [_V, _I, _I],
     "pip_time_steps": [_V, _I, _PD, _PD, _PD],
+    "pip_ens_create": [ctypes.POINTER(PipCfg), ctypes.POINTER(PipEnsMember), _I, _PP],
+    "pip_ens_upload": [_V, _I, _P, _P],
+    "pip_ens_download": [_V, _I, _P, _P],
+    "pip_ens_set_tile": [_V, _I],
+    "pip_ens_reset": [_V],
+    "pip_ens_run": [_V, _I, _PD],
+    "pip_ens_result": [_V, _I, _PD, _PL, _PI],
+    "pip_ens_destroy": [_V],
 }
 
 

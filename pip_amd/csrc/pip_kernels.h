@@ -221,34 +221,41 @@ __device__ __forceinline__ void block_reduce_store(RoundOut r, int nanflag, bool
   }
 }
 
-// ------------------------------------------------------------ stage kernel
+// ------------------------------------------------------------ stage tile
+// LDS footprint of one tile (doubles): primitives (TX+4)x(TY+4) x NVT, then
+// the face-flux tile NVT x max((TX+1)TY, TX(TY+1)).
+template <int NDIM, int NV, int TX, int TY>
+struct TileGeom {
+  static constexpr int NVT = NV + 5;
+  static constexpr int HX = TX + 4;
+  static constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
+  static constexpr int NFX = (TX + 1) * TY;
+  static constexpr int NFY = (NDIM == 2) ? TX * (TY + 1) : 0;
+  static constexpr int NF = (NFX > NFY) ? NFX : NFY;
+  static constexpr int LDS_DOUBLES = NVT * HY * HX + NVT * NF;
+};
+
+// One SSP-RK stage on the tile of interior cells [i0,i0+TX) x [j0,j0+TY)
+// (interior coordinates), executed by all TX*TY threads of the workgroup.
+// Returns whether the calling thread owns a cell; r / nanflag receive the
+// cell's dt terms when epi >= 1.
 template <int NDIM, int NV, bool IR, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
-  constexpr int NVT = NV + 5;
+__device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double cmax, int epi, int i0,
+                                           int j0, double* smem, RoundOut& r, int& nanflag) {
+  using G = TileGeom<NDIM, NV, TX, TY>;
+  constexpr int NVT = G::NVT;
   constexpr int NT = TX * TY;
-  constexpr int HX = TX + 4;
-  constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
+  constexpr int HX = G::HX;
+  constexpr int HY = G::HY;
   constexpr int OY = (NDIM == 2) ? 2 : 0;
-  constexpr int NFX = (TX + 1) * TY;
-  constexpr int NFY = (NDIM == 2) ? TX * (TY + 1) : 0;
-  constexpr int NF = (NFX > NFY) ? NFX : NFY;
-  extern __shared__ double smem[];
+  constexpr int NFX = G::NFX;
+  constexpr int NFY = G::NFY;
+  constexpr int NF = G::NF;
   double* V = smem;                      // NVT * HY * HX
   double* F = smem + NVT * HY * HX;      // NVT * NF
-
-  DevScalars* sc = a.sc;
-  if (!sc->active) return;
-  const double dt = sc->dt;
-  const double cmax = sc->cmax;
   double c0, c1, dt_sub;
   stage_coeffs(a.istep, a.t_order, dt, c0, c1, dt_sub);
-
   const int tid = threadIdx.x;
-  const int ntx = (a.nx + TX - 1) / TX;
-  const int bid = blockIdx.x;
-  const int bxi = bid % ntx, byi = bid / ntx;
-  const int i0 = bxi * TX;  // first interior column of the tile (interior coords)
-  const int j0 = byi * TY;
   const long long N = a.N;
   const int ix = a.ix;
 
@@ -392,8 +399,8 @@ __global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
     }
   }
 
-  RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
-  int nanflag = 0;
+  r = RoundOut{1.0e300, 0.0, 0.0, 0.0};
+  nanflag = 0;
   if (valid) {
     // add_source (Solver_rot.f90:498-503) for the five fluid variables
 #pragma unroll
@@ -405,19 +412,40 @@ __global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
       double fac = pexp(-dt_sub / dt * a.ph.safety * a.db_clean);
       un[8] = un[8] * fac;
     }
-    if (a.epi >= 1) {
+    if (epi >= 1) {
       double* um = un;
       double* uh = un + NV;
       r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
-      if (a.epi >= 2 && !sc->last) r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
+      if (epi >= 2) r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
 #pragma unroll
       for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
     }
     store_cell<NDIM, NVT>(a.Uout, N, ix, a.jx, ci, cj, un, a.bnd);
   }
+  return valid;
+}
+
+// ------------------------------------------------------------ stage kernel
+// One workgroup per tile.  The look-ahead round of the fused step (epi=2)
+// is skipped on the loop's last step (sc->last), as main.f90 ends there.
+template <int NDIM, int NV, bool IR, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
+  extern __shared__ double smem[];
+  DevScalars* sc = a.sc;
+  if (!sc->active) return;
+  const double dt = sc->dt;
+  const double cmax = sc->cmax;
+  const int epi = (a.epi >= 2 && sc->last) ? 1 : a.epi;
+  const int ntx = (a.nx + TX - 1) / TX;
+  const int bid = blockIdx.x;
+  const int i0 = (bid % ntx) * TX;
+  const int j0 = (bid / ntx) * TY;
+  RoundOut r;
+  int nanflag;
+  bool valid = stage_tile<NDIM, NV, IR, TX, TY>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
   if (a.epi >= 1) {
     __syncthreads();
-    block_reduce_store<NT>(r, nanflag, valid, sc, F);
+    block_reduce_store<TX * TY>(r, nanflag, valid, sc, smem);
   }
 }
 
@@ -596,5 +624,205 @@ __global__ void k_set_loop(DevScalars* sc, double tend, long long nmax, int acti
   sc->nmax = nmax;
   if (active >= 0) sc->active = active;
   if (ion_valid >= 0) sc->ion_pot_valid = ion_valid;
+}
+}  // namespace pip
+
+namespace pip {
+// ------------------------------------------------------ batched ensemble
+// One persistent launch integrates an ensemble of independent 1-D runs of
+// different sizes: each workgroup pulls a member index from a device queue
+// (longest expected run first) and runs that member's whole time loop
+// (main.f90:31-68) in a loop over TX-cell tiles of stage_tile, with its own
+// dt, time and step count.  The arithmetic per cell is that of the plain
+// path, so every member equals its plain run bit for bit.
+struct EnsMember {
+  long long off;  // offset (doubles) of the member's block: 13 planes of ix
+  int nx, ix;
+  double dx, col, gmin_safety, tend;
+  long long nmax;
+};
+struct EnsOut {
+  double time, dt;
+  long long nt;
+  int status, pad;
+};
+struct EnsArgs {
+  double* U;
+  double* U1;
+  const EnsMember* mem;
+  EnsOut* out;
+  const int* order;
+  unsigned* counter;
+  int n_members;
+  Phys ph;
+  Bnd bnd;
+  double db_clean;
+};
+
+__global__ void k_reset_counter(unsigned* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// Workgroup min/max of the dt terms; every thread gets the result.
+template <int NT>
+__device__ __forceinline__ RoundOut wg_reduce(RoundOut r, int& fl, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) {
+    r.dtc = dmin(r.dtc, __shfl_xor(r.dtc, off, 64));
+    r.rhoac = dmax(r.rhoac, __shfl_xor(r.rhoac, off, 64));
+    fl |= __shfl_xor(fl, off, 64);
+  }
+  constexpr int NW = (NT + 63) / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    lds[wid * 3 + 0] = r.dtc;
+    lds[wid * 3 + 1] = r.rhoac;
+    lds[wid * 3 + 2] = (double)fl;
+  }
+  __syncthreads();
+  RoundOut o = {lds[0], lds[1], 0.0, 0.0};
+  int f = (int)lds[2];
+  for (int w = 1; w < NW; w++) {
+    o.dtc = dmin(o.dtc, lds[w * 3 + 0]);
+    o.rhoac = dmax(o.rhoac, lds[w * 3 + 1]);
+    f |= (int)lds[w * 3 + 2];
+  }
+  __syncthreads();
+  fl = f;
+  return o;
+}
+
+// cfl rewrite round over all cells of a member (ghosts included), as
+// k_round does for the plain path; returns the reduced dt terms.
+template <int TX>
+__device__ __forceinline__ RoundOut ens_cfl_cells(double* U, const StageArgs& a, double* lds) {
+  RoundOut acc = {1.0e300, 0.0, 0.0, 0.0};
+  int fl = 0;
+  for (int k = threadIdx.x; k < a.ix; k += TX) {
+    double u[13];
+#pragma unroll
+    for (int n = 0; n < 13; n++) u[n] = U[n * a.N + k];
+    double dtc = cfl_cell<8>(u, u + 8, a.gmin_safety, a.ph);
+    bool interior = (k >= 2 && k < a.ix - 2);
+    if (interior) acc.dtc = dmin(acc.dtc, dtc);
+    acc.rhoac = dmax(acc.rhoac, dmax(u[0] * a.col, u[8] * a.col));
+#pragma unroll
+    for (int n = 0; n < 13; n++) U[n * a.N + k] = u[n];
+  }
+  return wg_reduce<TX>(acc, fl, lds);
+}
+
+__device__ __forceinline__ double ens_dt(const RoundOut& o, double safety) {
+  // cfl without IR: Scheme_rot.f90:186-201
+  double dttemp = dmin(10.0, o.dtc);
+  return dmin(dttemp, dmin(safety, 0.3) / o.rhoac);
+}
+
+template <int TX>
+__global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
+  extern __shared__ double smem[];
+  __shared__ int s_member;
+  using G = TileGeom<1, 8, TX, 1>;
+  double* lds_red = smem + G::LDS_DOUBLES;  // 3 * (TX/64) doubles past the tile
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_member = (int)atomicAdd(e.counter, 1u);
+    __syncthreads();
+    const int q = s_member;
+    if (q >= e.n_members) break;
+    const int m = e.order[q];
+    const EnsMember M = e.mem[m];
+    double* U = e.U + M.off;
+    double* U1 = e.U1 + M.off;
+    StageArgs a;
+    a.ac = nullptr;
+    a.arb_heat = nullptr;
+    a.ion_pot_out = nullptr;
+    a.sc = nullptr;
+    a.ix = M.ix;
+    a.jx = 1;
+    a.nx = M.nx;
+    a.ny = 1;
+    a.N = M.ix;
+    a.t_order = 2;
+    a.col = M.col;
+    a.gmin_safety = M.gmin_safety;
+    a.dxyz_min = M.dx;
+    a.dx = M.dx;
+    a.dy = 100.0;
+    a.db_clean = e.db_clean;
+    a.ph = e.ph;
+    a.irc = IRConst{0, 0, 0, 0, 0, 0, 0};
+    a.bnd = e.bnd;
+    // prologue: boundary map of the initial state (Initial_rot.f90:160)
+    if (threadIdx.x < 4) {
+      int g = (threadIdx.x < 2) ? threadIdx.x : M.ix - 4 + threadIdx.x;
+      bool flip;
+      int sidx = src_index(g, M.ix, e.bnd.type[0], e.bnd.type[1], flip);
+      unsigned msk = flip ? e.bnd.sym[(g < 2) ? 0 : 1] : 0u;
+      for (int n = 0; n < 13; n++) {
+        double v = U[n * a.N + sidx];
+        U[n * a.N + g] = ((msk >> n) & 1u) ? -v : v;
+      }
+    }
+    __syncthreads();
+    // prologue cfl, then the loop-top cfl of the first iteration
+    ens_cfl_cells<TX>(U, a, lds_red);
+    __syncthreads();
+    RoundOut o = ens_cfl_cells<TX>(U, a, lds_red);
+    double dt = ens_dt(o, e.ph.safety);
+    double time = 0.0;
+    long long nt = 0;
+    int status = 0;
+    while (time < M.tend && nt < M.nmax && status == 0) {
+      const bool last = (time + dt >= M.tend) || (nt + 1 >= M.nmax) || (dt <= 1.0e-12);
+      RoundOut r;
+      int nanflag;
+      a.istep = 1;
+      a.Uin = U;
+      a.U0 = U;
+      a.Uout = U1;
+      a.epi = 0;
+      for (int i0 = 0; i0 < M.nx; i0 += TX) {
+        stage_tile<1, 8, false, TX, 1>(a, dt, 0.0, 0, i0, 0, smem, r, nanflag);
+        __syncthreads();
+      }
+      __threadfence_block();
+      __syncthreads();
+      a.istep = 2;
+      a.Uin = U1;
+      a.U0 = U;
+      a.Uout = U;
+      const int epi = last ? 1 : 2;
+      RoundOut acc = {1.0e300, 0.0, 0.0, 0.0};
+      int fl = 0;
+      for (int i0 = 0; i0 < M.nx; i0 += TX) {
+        bool v = stage_tile<1, 8, false, TX, 1>(a, dt, 0.0, epi, i0, 0, smem, r, nanflag);
+        if (v) {
+          acc.dtc = dmin(acc.dtc, r.dtc);
+          acc.rhoac = dmax(acc.rhoac, r.rhoac);
+          fl |= nanflag;
+        }
+        __syncthreads();
+      }
+      __threadfence_block();
+      RoundOut red = wg_reduce<TX>(acc, fl, lds_red);
+      double dt_new = ens_dt(red, e.ph.safety);
+      time = time + dt;
+      nt = nt + 1;
+      if (dt <= 1.0e-12) status |= 1;
+      if (fl) status |= 2;
+      if (time < M.tend && nt < M.nmax && status == 0) dt = dt_new;
+    }
+    if (threadIdx.x == 0) {
+      EnsOut out;
+      out.time = time;
+      out.dt = dt;
+      out.nt = nt;
+      out.status = status;
+      out.pad = 0;
+      e.out[m] = out;
+    }
+  }
 }
 }  // namespace pip

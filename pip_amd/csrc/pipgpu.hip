@@ -9,7 +9,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/pipgpu.h"
 #include "pip_kernels.h"
@@ -615,6 +617,254 @@ int pip_time_steps(pip_ctx* c, int nsteps, double* ms_total, double* ms_stage1, 
   if (ms_stage2) *ms_stage2 = t2;
   c->top_pending = true;
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- ensemble
+struct pip_ens {
+  pip_cfg base;
+  int n;
+  std::vector<EnsMember> hmem;
+  EnsMember* mem = nullptr;
+  EnsOut* out = nullptr;
+  int* order = nullptr;
+  unsigned* counter = nullptr;
+  double* U = nullptr;
+  double* U1 = nullptr;
+  double* Uinit = nullptr;
+  long long total = 0;
+  hipStream_t stream = nullptr;
+  int tx = 256;
+  Phys ph;
+  Bnd bnd;
+  hipGraphExec_t gexec = nullptr;
+  int graph_tx = 0;
+};
+
+template <int TX>
+static int ens_launch(pip_ens* e) {
+  using G = TileGeom<1, 8, TX, 1>;
+  size_t lds = sizeof(double) * (size_t)(G::LDS_DOUBLES + 3 * (TX / 64));
+  static bool attr = false;
+  auto kern = k_ensemble<TX>;
+  if (!attr) {
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  int dev = 0, ncu = 0, nb = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)kern, TX, lds));
+  if (nb < 1) return fail(PIP_ERR_HIP, "ensemble kernel does not fit on a CU");
+  int grid = ncu * nb;
+  if (grid > e->n) grid = e->n;
+  EnsArgs a;
+  a.U = e->U;
+  a.U1 = e->U1;
+  a.mem = e->mem;
+  a.out = e->out;
+  a.order = e->order;
+  a.counter = e->counter;
+  a.n_members = e->n;
+  a.ph = e->ph;
+  a.bnd = e->bnd;
+  a.db_clean = e->base.db_clean;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, e->stream, e->counter);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(TX), lds, e->stream, a);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+static int ens_enqueue(pip_ens* e) {
+  switch (e->tx) {
+    case 64: return ens_launch<64>(e);
+    case 128: return ens_launch<128>(e);
+    case 256: return ens_launch<256>(e);
+    case 512: return ens_launch<512>(e);
+  }
+  return fail(PIP_ERR_ARG, "ensemble tile must be 64, 128, 256 or 512");
+}
+
+extern "C" {
+
+int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pip_ens** out) {
+  if (!base || !members || !out || n <= 0) return fail(PIP_ERR_ARG, "null argument or n <= 0");
+  *out = nullptr;
+  if (base->ndim != 1 || base->nvar_m != 8) return fail(PIP_ERR_UNSUPPORTED, "ensemble members are 1-D (nvar_m = 8)");
+  if (base->flag_ir != 0 || base->ac_field) return fail(PIP_ERR_UNSUPPORTED, "ensemble: flag_IR = 0, constant alpha_c");
+  if (base->t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "ensemble: t_order = 2");
+  for (int s = 0; s < 2; s++) {
+    int t = bnd_type(base->flag_bnd[s]);
+    if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
+  }
+  for (int m = 0; m < n; m++)
+    if (members[m].nx < 4 || !(members[m].dx > 0)) return fail(PIP_ERR_ARG, "member nx >= 4 and dx > 0 required");
+  pip_ens* e = new pip_ens();
+  e->base = *base;
+  e->n = n;
+  e->ph.gm = base->gm;
+  e->ph.ro_lim = base->ro_lim;
+  e->ph.pr_lim = base->pr_lim;
+  e->ph.safety = base->safety;
+  e->ph.T0 = base->T0;
+  e->ph.n0 = base->n0;
+  e->ph.beta = base->beta;
+  e->ph.t_ir = base->t_ir;
+  e->ph.flag_col = base->flag_col;
+  e->ph.flag_ir = 0;
+  e->ph.flag_ir_type = 0;
+  {  // boundary masks of a 1-D, nvar_m = 8 context
+    pip_ctx tmp;
+    tmp.cfg = *base;
+    tmp.nv = 8;
+    build_bnd(&tmp);
+    e->bnd = tmp.bnd;
+  }
+  e->hmem.resize(n);
+  long long off = 0;
+  for (int m = 0; m < n; m++) {
+    EnsMember& M = e->hmem[m];
+    M.off = off;
+    M.nx = members[m].nx;
+    M.ix = members[m].nx + 4;
+    M.dx = members[m].dx;
+    M.col = members[m].col;
+    M.gmin_safety = base->safety * dmin(dmin(M.dx, 100.0), 100.0);
+    M.tend = members[m].tend;
+    M.nmax = members[m].max_steps > 0 ? members[m].max_steps : ((long long)1 << 62);
+    off += 13LL * M.ix;
+  }
+  e->total = off;
+  std::vector<int> order(n);
+  for (int m = 0; m < n; m++) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    double ca = members[a].cost > 0 ? members[a].cost : (double)members[a].nx;
+    double cb = members[b].cost > 0 ? members[b].cost : (double)members[b].nx;
+    return ca > cb;
+  });
+  hipError_t er;
+  auto bail = [&](const char* what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorString(er);
+    pip_ens_destroy(e);
+    return fail(PIP_ERR_HIP, msg);
+  };
+  if ((er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream");
+  if ((er = hipMalloc(&e->U, sizeof(double) * off)) != hipSuccess) return bail("hipMalloc U");
+  if ((er = hipMalloc(&e->U1, sizeof(double) * off)) != hipSuccess) return bail("hipMalloc U1");
+  if ((er = hipMalloc(&e->Uinit, sizeof(double) * off)) != hipSuccess) return bail("hipMalloc Uinit");
+  if ((er = hipMalloc(&e->mem, sizeof(EnsMember) * n)) != hipSuccess) return bail("hipMalloc mem");
+  if ((er = hipMalloc(&e->out, sizeof(EnsOut) * n)) != hipSuccess) return bail("hipMalloc out");
+  if ((er = hipMalloc(&e->order, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc order");
+  if ((er = hipMalloc(&e->counter, sizeof(unsigned) * 4)) != hipSuccess) return bail("hipMalloc counter");
+  if ((er = hipMemcpy(e->mem, e->hmem.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
+  if ((er = hipMemcpy(e->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
+  if ((er = hipMemset(e->U, 0, sizeof(double) * off)) != hipSuccess) return bail("memset");
+  if ((er = hipMemset(e->U1, 0, sizeof(double) * off)) != hipSuccess) return bail("memset");
+  if ((er = hipMemset(e->Uinit, 0, sizeof(double) * off)) != hipSuccess) return bail("memset");
+  if ((er = hipMemset(e->out, 0, sizeof(EnsOut) * n)) != hipSuccess) return bail("memset");
+  *out = e;
+  return PIP_OK;
+}
+
+int pip_ens_destroy(pip_ens* e) {
+  if (!e) return PIP_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  (void)hipFree(e->U);
+  (void)hipFree(e->U1);
+  (void)hipFree(e->Uinit);
+  (void)hipFree(e->mem);
+  (void)hipFree(e->out);
+  (void)hipFree(e->order);
+  (void)hipFree(e->counter);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return PIP_OK;
+}
+
+int pip_ens_upload(pip_ens* e, int m, const double* U_h, const double* U_m) {
+  if (!e || !U_h || !U_m || m < 0 || m >= e->n) return fail(PIP_ERR_ARG, "bad member or null array");
+  const EnsMember& M = e->hmem[m];
+  size_t pb = sizeof(double) * M.ix;
+  HIPCHK(hipMemcpy(e->Uinit + M.off, U_m, pb * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->Uinit + M.off + 8LL * M.ix, U_h, pb * 5, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->U + M.off, e->Uinit + M.off, pb * 13, hipMemcpyDeviceToDevice));
+  return PIP_OK;
+}
+
+int pip_ens_download(pip_ens* e, int m, double* U_h, double* U_m) {
+  if (!e || !U_h || !U_m || m < 0 || m >= e->n) return fail(PIP_ERR_ARG, "bad member or null array");
+  const EnsMember& M = e->hmem[m];
+  size_t pb = sizeof(double) * M.ix;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(U_m, e->U + M.off, pb * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(U_h, e->U + M.off + 8LL * M.ix, pb * 5, hipMemcpyDeviceToHost));
+  return PIP_OK;
+}
+
+int pip_ens_set_tile(pip_ens* e, int tx) {
+  if (!e) return fail(PIP_ERR_ARG, "null ensemble");
+  if (tx != 64 && tx != 128 && tx != 256 && tx != 512) return fail(PIP_ERR_ARG, "tile must be 64/128/256/512");
+  e->tx = tx;
+  return PIP_OK;
+}
+
+int pip_ens_reset(pip_ens* e) {
+  if (!e) return fail(PIP_ERR_ARG, "null ensemble");
+  HIPCHK(hipMemcpyAsync(e->U, e->Uinit, sizeof(double) * e->total, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return PIP_OK;
+}
+
+int pip_ens_run(pip_ens* e, int use_graph, double* ms) {
+  if (!e) return fail(PIP_ERR_ARG, "null ensemble");
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  if (use_graph) {
+    if (!e->gexec || e->graph_tx != e->tx) {
+      if (e->gexec) {
+        HIPCHK(hipGraphExecDestroy(e->gexec));
+        e->gexec = nullptr;
+      }
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int rc = ens_enqueue(e);
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (rc != PIP_OK) return rc;
+      if (ce != hipSuccess) return fail(PIP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+      HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      e->graph_tx = e->tx;
+    }
+    HIPCHK(hipEventRecord(a, e->stream));
+    HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    HIPCHK(hipEventRecord(b, e->stream));
+  } else {
+    HIPCHK(hipEventRecord(a, e->stream));
+    int rc = ens_enqueue(e);
+    if (rc != PIP_OK) return rc;
+    HIPCHK(hipEventRecord(b, e->stream));
+  }
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (ms) *ms = t;
+  return PIP_OK;
+}
+
+int pip_ens_result(pip_ens* e, int m, double* time, long* nt, int* status) {
+  if (!e || m < 0 || m >= e->n) return fail(PIP_ERR_ARG, "bad member");
+  EnsOut o;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(&o, e->out + m, sizeof(o), hipMemcpyDeviceToHost));
+  if (time) *time = o.time;
+  if (nt) *nt = (long)o.nt;
+  if (status) *status = o.status;
+  return PIP_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,54 @@
+"""Batched ensemble (one persistent launch, optional hipGraph replay) equals
+the plain per-member runs bit for bit."""
+import numpy as np
+import pytest
+
+from pip_amd import problems
+from pip_amd.ensemble import Ensemble
+from pip_amd.solver import PIPSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(n, nx_cap=None):
+    params = problems.ensemble_members(256, seed=7)[:n]
+    out = []
+    for p in params:
+        if nx_cap:
+            p = dict(p, nx=min(p["nx"], nx_cap))
+        cfg, uh, um, _ = problems.ensemble_member(p)
+        out.append((cfg, uh, um))
+    return out
+
+
+@pytest.mark.parametrize("graph,tile", [(True, 256), (False, 64), (True, 512)])
+def test_ensemble_matches_plain_runs(graph, tile):
+    print("case ensemble 10 members graph", graph, "tile", tile, flush=True)
+    mem = _members(10)
+    ens = Ensemble(mem, tend=0.1, tile=tile)
+    ms = ens.run(graph=graph)
+    print("ensemble ms", ms, flush=True)
+    for k, (cfg, uh, um) in enumerate(mem):
+        s = PIPSolver(cfg, uh, um)
+        n, t = s.advance(tend=0.1)
+        r = ens.result(k)
+        assert r["status"] == 0
+        assert r["nt"] == n and r["time"] == t, (k, r, n, t)
+        eh, em = ens.state(k)
+        ph, pm = s.state()
+        assert np.array_equal(em[:, :, 2:-2], pm[:, :, 2:-2]), k
+        assert np.array_equal(eh[:, :, 2:-2], ph[:, :, 2:-2]), k
+        s.close()
+
+
+def test_ensemble_rerun_after_reset_is_identical():
+    print("case ensemble reset/rerun", flush=True)
+    mem = _members(6, nx_cap=600)
+    ens = Ensemble(mem, tend=0.05)
+    ens.run(graph=True)
+    a = [ens.state(k) for k in range(6)]
+    ens.reset()
+    ens.run(graph=True)
+    for k in range(6):
+        b = ens.state(k)
+        assert np.array_equal(a[k][0], b[0]) and np.array_equal(a[k][1], b[1])
