@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=str, default="auto", help="TXxTY, or auto (autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -65,11 +66,15 @@ def main():
         sys.exit(2)
 
     import torch  # device plumbing only: sync around the timed region
-    from pip_amd import problems
+    from pip_amd import autotune, problems
     from pip_amd.solver import PIPSolver
 
     cfg, uh, um, ac = problems.orszag_tang(args.n)
-    s = PIPSolver(cfg, uh, um, ac)
+    if args.tile == "auto":
+        tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
+    else:
+        tile = tuple(int(v) for v in args.tile.split("x"))
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.enqueue(args.warmup)
     s.sync()
     torch.cuda.synchronize()
@@ -104,7 +109,7 @@ def main():
         "data": "synthetic (Orszag-Tang initial state, no checkpoint)",
         "config": {"workload": "orszag_tang_2d_two_fluid", "nx": args.n, "ny": args.n,
                    "nvar": nv, "xi_n": 0.9, "alpha_c": 1.0e3, "flag_divb": 1,
-                   "ssp_rk": 2, "cfl": 0.3, "tile": [s_tile for s_tile in (32, 8)]},
+                   "ssp_rk": 2, "cfl": 0.3, "tile": list(tile)},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "kernel": "k_stage (stage 2)", "algorithmic_bytes_per_launch": bytes_stage2,

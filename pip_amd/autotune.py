@@ -11,7 +11,7 @@ import os
 from . import build
 
 CACHE = os.path.join(build.LIBDIR, "autotune.json")
-STAGE_TILES = [(32, 8), (16, 16), (64, 4), (32, 4), (16, 8)]
+STAGE_TILES = [(16, 15), (32, 7), (32, 15), (32, 8), (16, 7)]
 ENS_TILES = [64, 128, 256, 512]
 
 

@@ -1,0 +1,444 @@
+"""Generates pip_adj_gen.h: reverse-mode (vector-Jacobian) code for the
+face fluxes and the coupling source of the two-fluid step.
+
+The forward computations are restated here as SSA statements (one
+assignment per intermediate) organised in a tree of branches that mirrors the
+reference (HLL_rot.f90:615-687 HLLC, :691-910 HLLD with the HLLC-G fallback
+and the GLM psi flux, PIP_rot.f90:1359-1368 collision source).  For every
+statement y = f(x1..xk) the generator emits the local chain rule
+x_i_bar += y_bar * df/dx_i (derivatives by sympy), in reverse order, after
+re-running the forward statements of the taken branch.  Branch decisions
+(wave-speed ordering, HLLD region, min/max, floors) are those of the forward
+point: the result is the exact derivative of the piecewise-smooth map at
+that point.
+
+Run:  python pip_amd/csrc/gen_adjoint.py  (rewrites pip_adj_gen.h)
+"""
+from __future__ import annotations
+
+import os
+
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class Printer(C99CodePrinter):
+    def _print_Pow(self, e):
+        b, x = e.base, e.exp
+        pb = self.parenthesize(b, 100)
+        if x == 2:
+            return f"({pb}*{pb})"
+        if x == 3:
+            return f"({pb}*{pb}*{pb})"
+        if x == -1:
+            return f"(1.0/{pb})"
+        if x == -2:
+            return f"(1.0/({pb}*{pb}))"
+        if x == sp.Rational(1, 2):
+            return f"sqrt({self._print(b)})"
+        if x == sp.Rational(-1, 2):
+            return f"(1.0/sqrt({self._print(b)}))"
+        if x == sp.Rational(-3, 2):
+            return f"(1.0/({pb}*sqrt({self._print(b)})))"
+        return super()._print_Pow(e)
+
+    def _print_Rational(self, e):
+        return f"({float(e.p)}/{float(e.q)})"
+
+    def _print_Integer(self, e):
+        return f"{float(e)}"
+
+    def _print_Float(self, e):
+        return repr(float(e))
+
+    def _print_sign(self, e):
+        a = self._print(e.args[0])
+        return f"(({a}) > 0.0 ? 1.0 : (({a}) < 0.0 ? -1.0 : 0.0))"
+
+
+PR = Printer()
+
+
+def cc(e) -> str:
+    return PR.doprint(e)
+
+
+class Block:
+    """A straight-line sequence of statements plus optional branches."""
+
+    def __init__(self, g):
+        self.g = g
+        self.items = []  # ('let', sym, expr, kind) | ('if', [(cond, Block), ...])
+
+    def let(self, name, expr, kind="expr"):
+        s = self.g.sym(name)
+        self.items.append(("let", s, sp.sympify(expr), kind))
+        return s
+
+    def out(self, name, expr):
+        """Assign an output symbol (may be assigned in several branches)."""
+        s = self.g.syms.get(name) or self.g.sym(name)
+        self.items.append(("let", s, sp.sympify(expr), "expr"))
+        return s
+
+    def minv(self, name, a, b):
+        return self.let(name, sp.Min(a, b, evaluate=False), "min")
+
+    def maxv(self, name, a, b):
+        return self.let(name, sp.Max(a, b, evaluate=False), "max")
+
+    def const(self, name, code):
+        """A value with zero derivative (e.g. sign), given as C code."""
+        s = self.g.sym(name)
+        self.items.append(("let", s, code, "const"))
+        return s
+
+    def branches(self, *conds):
+        """conds: C conditions; the last may be None (else).  Returns blocks."""
+        blocks = [Block(self.g) for _ in conds]
+        self.items.append(("if", list(zip(conds, blocks))))
+        return blocks
+
+
+class Gen:
+    def __init__(self, name):
+        self.name = name
+        self.syms = {}
+        self.inputs = []
+        self.root = Block(self)
+
+    def sym(self, name):
+        if name in self.syms:
+            raise ValueError("redefined " + name)
+        s = sp.Symbol(name, real=True)
+        self.syms[name] = s
+        return s
+
+    def inp(self, name):
+        s = self.sym(name)
+        self.inputs.append(s)
+        return s
+
+    # ------------------------------------------------------------ emission
+    def _fwd(self, blk, out, ind):
+        pad = " " * ind
+        for it in blk.items:
+            if it[0] == "let":
+                _, s, e, kind = it
+                if kind == "const":
+                    out.append(f"{pad}{s} = {e};")
+                elif kind == "min":
+                    a, b = e.args
+                    out.append(f"{pad}{s} = ({cc(a)} < {cc(b)}) ? {cc(a)} : {cc(b)};")
+                elif kind == "max":
+                    a, b = e.args
+                    out.append(f"{pad}{s} = ({cc(a)} > {cc(b)}) ? {cc(a)} : {cc(b)};")
+                else:
+                    out.append(f"{pad}{s} = {cc(e)};")
+            else:
+                pass  # branches handled by _emit
+
+    def _rev_stmts(self, blk, out, ind):
+        pad = " " * ind
+        for it in reversed(blk.items):
+            if it[0] != "let":
+                continue
+            _, s, e, kind = it
+            if kind == "const":
+                continue
+            ab = f"a_{s}"
+            if kind in ("min", "max"):
+                a, b = e.args
+                op = "<" if kind == "min" else ">"
+                ta = f"a_{a}" if isinstance(a, sp.Symbol) and a in self.syms.values() else None
+                tb = f"a_{b}" if isinstance(b, sp.Symbol) and b in self.syms.values() else None
+                out.append(f"{pad}if ({cc(a)} {op} {cc(b)}) {{ {ta + ' += ' + ab + ';' if ta else ''} }}"
+                           f" else {{ {tb + ' += ' + ab + ';' if tb else ''} }}")
+                continue
+            for x in sorted(e.free_symbols, key=lambda q: q.name):
+                if x not in self.syms.values():
+                    continue
+                d = sp.diff(e, x)
+                if d == 0:
+                    continue
+                out.append(f"{pad}a_{x} += {ab} * ({cc(d)});")
+
+    def _emit(self, blk, out, ind):
+        """forward of blk, then its branches (each fully emitted), then the
+        reverse of blk's own statements."""
+        pad = " " * ind
+        # forward statements before the first branch, branches, reverse
+        self._fwd(blk, out, ind)
+        for it in blk.items:
+            if it[0] == "if":
+                first = True
+                for cond, child in it[1]:
+                    if cond is None:
+                        out.append(f"{pad}}} else {{")
+                    elif first:
+                        out.append(f"{pad}if ({cond}) {{")
+                    else:
+                        out.append(f"{pad}}} else if ({cond}) {{")
+                    first = False
+                    self._emit(child, out, ind + 2)
+                out.append(f"{pad}}}")
+        self._rev_stmts(blk, out, ind)
+
+    def all_syms(self):
+        return list(self.syms.values())
+
+    def emit(self, signature, prologue, epilogue, out_seeds):
+        """out_seeds: list of (symbol name, C expression of its adjoint)."""
+        body = []
+        inter = [s for s in self.all_syms() if s not in self.inputs]
+        body.append("  // forward intermediates")
+        for i in range(0, len(inter), 8):
+            body.append("  double " + ", ".join(str(s) for s in inter[i:i + 8]) + ";")
+        body.append("  // adjoints")
+        allv = self.all_syms()
+        for i in range(0, len(allv), 6):
+            body.append("  double " + ", ".join(f"a_{s} = 0.0" for s in allv[i:i + 6]) + ";")
+        for name, expr in out_seeds:
+            body.append(f"  a_{name} = {expr};")
+        self._emit(self.root, body, 2)
+        return signature + " {\n" + prologue + "\n".join(body) + "\n" + epilogue + "}\n"
+
+
+# ====================================================================== HLLC
+def gen_hllc():
+    g = Gen("hllc")
+    vl = [g.inp(f"vl{k}") for k in range(5)]
+    vr = [g.inp(f"vr{k}") for k in range(5)]
+    gm = sp.Symbol("gm", real=True)
+    B = g.root
+    acc = sp.Float(1.03)
+    epsi = 1.0e-10
+
+    def side(v, t):
+        u0 = B.let(f"u{t}0", v[0])
+        u1 = B.let(f"u{t}1", v[0] * v[1])
+        u2 = B.let(f"u{t}2", v[0] * v[2])
+        u3 = B.let(f"u{t}3", v[0] * v[3])
+        u4 = B.let(f"u{t}4", v[4] / (gm - 1) + sp.Rational(1, 2) * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2))
+        u = [u0, u1, u2, u3, u4]
+        p = v[4]
+        f = [B.let(f"f{t}0", u1), B.let(f"f{t}1", p + u1 ** 2 / u0), B.let(f"f{t}2", u1 * u2 / u0),
+             B.let(f"f{t}3", u1 * u3 / u0), B.let(f"f{t}4", u1 * (u4 + p) / u0)]
+        cs = B.let(f"cs{t}", sp.sqrt(gm * p / u0))
+        vn = B.let(f"vn{t}", u1 / u0)
+        return u, f, cs, vn, p
+
+    ul, fl, csl, vnl, pl = side(vl, "l")
+    ur, fr, csr, vnr, pr = side(vr, "r")
+    a1 = B.let("sla", vnl - csl)
+    a2 = B.let("slb", vnr - csr)
+    mn = B.minv("slm", a1, a2)
+    sl = B.let("sl", acc * mn)
+    b1 = B.let("sra", vnl + csl)
+    b2 = B.let("srb", vnr + csr)
+    mx = B.maxv("srm", b1, b2)
+    sr = B.let("sr", acc * mx)
+    sm = B.let("sm", (ur[1] * (sr - vnr) - ul[1] * (sl - vnl) + pl - pr) / (ur[0] * (sr - vnr) - ul[0] * (sl - vnl)))
+    prm = B.let("prm", sm * ul[0] * (sl - vnl) - ul[1] * (sl - vnl) + pl)
+    stl0 = B.let("stl0", ul[0] * (sl - vnl) / (sl - sm))
+    str0 = B.let("str0", ur[0] * (sr - vnr) / (sr - sm))
+    stl = [stl0, B.let("stl1", stl0 * sm), B.let("stl2", stl0 * ul[2] / ul[0]),
+           B.let("stl3", stl0 * ul[3] / ul[0]),
+           B.let("stl4", ((sl - vnl) * ul[4] - pl * vnl + prm * sm) / (sl - sm))]
+    st_r = [str0, B.let("str1", str0 * sm), B.let("str2", str0 * ur[2] / ur[0]),
+            B.let("str3", str0 * ur[3] / ur[0]),
+            B.let("str4", ((sr - vnr) * ur[4] - pr * vnr + prm * sm) / (sr - sm))]
+    # smooth switches (HLL_rot.f90:679-681): derivative inside the epsi band
+    # is kept; outside it they are constants 0/1
+    thl = B.let("thl", (sp.Abs(sl) + sl) / sp.Max(2 * sl, epsi))
+    thm = B.let("thm", (sp.Abs(sm) + sm) / sp.Max(2 * sm, epsi))
+    thr = B.let("thr", 1 - (sp.Abs(sr) + sr) / sp.Max(2 * sr, epsi))
+    for k in range(5):
+        B.let(f"F{k}", thr * fr[k] + thl * fl[k] + (1 - thl - thr) *
+              (thm * (fl[k] + sl * (stl[k] - ul[k])) + (1 - thm) * (fr[k] + sr * (st_r[k] - ur[k]))))
+    sig = "PIP_HD void hllc_flux_vjp(const double* vl, const double* vr, double gm, const double* lam, double* lvl, double* lvr)"
+    pro = "".join(f"  const double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(5))
+    epi = "".join(f"  lvl[{k}] += a_vl{k};\n  lvr[{k}] += a_vr{k};\n" for k in range(5))
+    seeds = [(f"F{k}", f"lam[{k}]") for k in range(5)]
+    return g.emit(sig, pro, epi, seeds)
+
+
+# ====================================================================== HLLD
+def gen_hlld():
+    g = Gen("hlld")
+    NV = 9
+    vl = [g.inp(f"vl{k}") for k in range(NV)]
+    vr = [g.inp(f"vr{k}") for k in range(NV)]
+    gm = sp.Symbol("gm", real=True)
+    ch = sp.Symbol("ch", real=True)
+    h = sp.Float(1.001)
+    R = g.root
+    half = sp.Rational(1, 2)
+
+    def side(v, t):
+        b2 = R.let(f"b2{t}", v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
+        u = [v[0], R.let(f"u{t}1", v[0] * v[1]), R.let(f"u{t}2", v[0] * v[2]), R.let(f"u{t}3", v[0] * v[3]),
+             R.let(f"u{t}4", v[4] / (gm - 1) + half * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) + half * b2),
+             v[5], v[6], v[7], v[8]]
+        pt = R.let(f"pt{t}", v[4] + half * b2)
+        vb = R.let(f"vb{t}", v[1] * v[5] + v[2] * v[6] + v[3] * v[7])
+        f = [R.let(f"f{t}0", v[0] * v[1]),
+             R.let(f"f{t}1", v[0] * v[1] ** 2 + v[4] + half * b2 - v[5] ** 2),
+             R.let(f"f{t}2", v[0] * v[1] * v[2] - v[5] * v[6]),
+             R.let(f"f{t}3", v[0] * v[1] * v[3] - v[5] * v[7]),
+             R.let(f"f{t}4", (u[4] + pt) * v[1] - vb * v[5]),
+             sp.Integer(0),
+             R.let(f"f{t}6", v[1] * v[6] - v[5] * v[2]),
+             R.let(f"f{t}7", v[1] * v[7] - v[5] * v[3])]
+        f1 = R.let(f"gp{t}", gm * v[4])
+        A = R.let(f"A{t}", f1 + b2)
+        Dr = R.let(f"Dr{t}", A ** 2 - 4 * f1 * v[5] ** 2)
+        D = R.maxv(f"D{t}", Dr, sp.Float(0.0))
+        vf = R.let(f"vf{t}", sp.sqrt((A + sp.sqrt(D)) / (2 * v[0])))
+        return u, f, pt, vb, vf
+
+    ul, fl, ptl, vbl, vfl = side(vl, "l")
+    ur, fr, ptr, vbr, vfr = side(vr, "r")
+    mnv = R.minv("mnv", vl[1], vr[1])
+    mxv = R.maxv("mxv", vl[1], vr[1])
+    mxf = R.maxv("mxf", vfl, vfr)
+    sl = R.let("sl", mnv - mxf)
+    sr = R.let("sr", mxv + mxf)
+
+    def glm(blk, outs):
+        # flux components 5 and 8 overridden by the GLM terms (HLL_rot.f90:904-908)
+        outs[5] = half * (vl[8] + vr[8] - ch * (vr[5] - vl[5]))
+        outs.append(half * (ch * ch * (vl[5] + vr[5]) - ch * (vr[8] - vl[8])))
+        for k in range(9):
+            blk.out(f"F{k}", outs[k])
+
+    bA, bB, bC = R.branches("sl >= 0.0", "sr <= 0.0", None)
+    glm(bA, list(fl))
+    glm(bB, list(fr))
+    inv = bC.let("inv", 1 / (sr - sl))
+    uh = [bC.let(f"uh{k}", inv * (sr * ur[k] - sl * ul[k] - fr[k] + fl[k])) for k in range(8)]
+    sm = bC.let("sm", uh[1] / uh[0])
+    pt = bC.let("pt", ptl + vl[0] * (sl - vl[1]) * (sm - vl[1]))
+    rol1 = bC.let("rol1", vl[0] * (sl - vl[1]) / (sl - sm))
+    ror1 = bC.let("ror1", vr[0] * (sr - vr[1]) / (sr - sm))
+    sal = bC.let("sal", sp.Abs(uh[5]) / sp.sqrt(rol1))
+    sar = bC.let("sar", sp.Abs(uh[5]) / sp.sqrt(ror1))
+    cG, cD = bC.branches("(sl >= (sm - 1.001 * sal)) || ((sm + 1.001 * sar) >= sr)", None)
+    # ---- HLLC-G fallback
+    vy2 = cG.let("gvy2", uh[2] / uh[0])
+    vz2 = cG.let("gvz2", uh[3] / uh[0])
+    gL, gR = cG.branches("sm >= 0.0", None)
+    for blk, v, u, f, s, ro1, ptk, vb, t in ((gL, vl, ul, fl, sl, rol1, ptl, vbl, "L"),
+                                             (gR, vr, ur, fr, sr, ror1, ptr, vbr, "R")):
+        u24 = blk.let(f"g{t}u24", ((s - v[1]) * u[4] - ptk * v[1] + pt * sm +
+                                   uh[5] * (vb - sm * uh[5] - vy2 * uh[6] - vz2 * uh[7])) / (s - sm))
+        u2 = [None, ro1 * sm, ro1 * vy2, ro1 * vz2, u24]
+        outs = [s * (ro1 - v[0]) + f[0]] + [s * (u2[k] - u[k]) + f[k] for k in range(1, 5)] + \
+               [s * (uh[k] - v[k]) + f[k] for k in range(5, 8)]
+        glm(blk, outs)
+    # ---- HLLD
+    bh2 = cD.let("bh2", uh[5] ** 2)
+
+    def star(v, u, s, ro1, ptk, vb, t):
+        f1 = cD.let(f"d{t}f1", 1 / (v[0] * (s - v[1]) * (s - sm) - bh2))
+        e = cD.let(f"d{t}e", v[0] * (s - v[1]) ** 2 - bh2)
+        u16 = cD.let(f"d{t}u16", v[6] * f1 * e)
+        u17 = cD.let(f"d{t}u17", v[7] * f1 * e)
+        vy1 = cD.let(f"d{t}vy1", v[2] - f1 * uh[5] * v[6] * (sm - v[1]))
+        vz1 = cD.let(f"d{t}vz1", v[3] - f1 * uh[5] * v[7] * (sm - v[1]))
+        u14 = cD.let(f"d{t}u14", ((s - v[1]) * u[4] - ptk * v[1] + pt * sm +
+                                  uh[5] * (vb - sm * uh[5] - vy1 * u16 - vz1 * u17)) / (s - sm))
+        u1 = [ro1, ro1 * sm, ro1 * vy1, ro1 * vz1, u14, uh[5], u16, u17]
+        return u1, vy1, vz1
+
+    ul1, vyl1, vzl1 = star(vl, ul, sl, rol1, ptl, vbl, "L")
+    ur1, vyr1, vzr1 = star(vr, ur, sr, ror1, ptr, vbr, "R")
+    sl1 = cD.let("sl1", sm - sal)
+    sr1 = cD.let("sr1", sm + sar)
+    dL, dR, dC = cD.branches("sl1 >= 0.0", "sr1 <= 0.0", None)
+    for blk, v, u, f, s, u1 in ((dL, vl, ul, fl, sl, ul1), (dR, vr, ur, fr, sr, ur1)):
+        outs = [s * (u1[0] - v[0]) + f[0]] + [s * (u1[k] - u[k]) + f[k] for k in range(1, 5)] + \
+               [s * (u1[k] - v[k]) + f[k] for k in range(5, 8)]
+        glm(blk, outs)
+    rls = dC.let("rls", sp.sqrt(rol1))
+    rrs = dC.let("rrs", sp.sqrt(ror1))
+    fc = dC.let("fc", 1 / (rls + rrs))
+    sg = dC.const("sg", "copysign(1.0, uh5)")
+    cvy2 = dC.let("cvy2", fc * (rls * vyl1 + rrs * vyr1 + (ur1[6] - ul1[6]) * sg))
+    cvz2 = dC.let("cvz2", fc * (rls * vzl1 + rrs * vzr1 + (ur1[7] - ul1[7]) * sg))
+    u26 = dC.let("u26", fc * (rls * ur1[6] + rrs * ul1[6] + rls * rrs * (vyr1 - vyl1) * sg))
+    u27 = dC.let("u27", fc * (rls * ur1[7] + rrs * ul1[7] + rls * rrs * (vzr1 - vzl1) * sg))
+    cL, cR = dC.branches("sm >= 0.0", None)
+    for blk, v, u, f, s, s1, u1, ros, vy1, vz1, sgn, t in (
+            (cL, vl, ul, fl, sl, sl1, ul1, rls, vyl1, vzl1, -1, "L"),
+            (cR, vr, ur, fr, sr, sr1, ur1, rrs, vyr1, vzr1, 1, "R")):
+        u24 = blk.let(f"c{t}u24", u1[4] + sgn * ros * (vy1 * u1[6] + vz1 * u1[7] - cvy2 * u26 - cvz2 * u27) * sg)
+        u2 = [u1[0], u1[0] * sm, u1[0] * cvy2, u1[0] * cvz2, u24, uh[5], u26, u27]
+        outs = [s1 * u2[0] - (s1 - s) * u1[0] - s * v[0] + f[0]] + \
+               [s1 * u2[k] - (s1 - s) * u1[k] - s * u[k] + f[k] for k in range(1, 5)] + \
+               [s1 * u2[k] - (s1 - s) * u1[k] - s * v[k] + f[k] for k in range(5, 8)]
+        glm(blk, outs)
+    sig = ("PIP_HD void hlld9_flux_vjp(const double* vl, const double* vr, double gm, double ch, "
+           "const double* lam, double* lvl, double* lvr)")
+    pro = "".join(f"  const double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(9))
+    epi = "".join(f"  lvl[{k}] += a_vl{k};\n  lvr[{k}] += a_vr{k};\n" for k in range(9))
+    seeds = [(f"F{k}", f"lam[{k}]") for k in range(9)]
+    return g.emit(sig, pro, epi, seeds)
+
+
+# ================================================================= source
+def gen_source():
+    """Collision source (explicit, PIP_rot.f90:1359-1368) as a function of the
+    conserved states and alpha; the cq2pv rewrites are identities of the
+    state (floors inactive).  Outputs S_m[0..4]; S_h = -S_m."""
+    g = Gen("src")
+    um = [g.inp(f"um{k}") for k in range(8)]
+    uh = [g.inp(f"uh{k}") for k in range(5)]
+    ac = g.inp("ac")
+    gm = sp.Symbol("gm", real=True)
+    B = g.root
+    half = sp.Rational(1, 2)
+    nde = uh[0]
+    nvx, nvy, nvz = B.let("nvx", uh[1] / nde), B.let("nvy", uh[2] / nde), B.let("nvz", uh[3] / nde)
+    npr = B.let("npr", (gm - 1) * (uh[4] - half * nde * (nvx ** 2 + nvy ** 2 + nvz ** 2)))
+    de = um[0]
+    vx, vy, vz = B.let("vx", um[1] / de), B.let("vy", um[2] / de), B.let("vz", um[3] / de)
+    pr = B.let("pr", (gm - 1) * (um[4] - half * de * (vx ** 2 + vy ** 2 + vz ** 2) -
+                                half * (um[5] ** 2 + um[6] ** 2 + um[7] ** 2)))
+    te = B.let("te", pr / de * gm * half)
+    nte = B.let("nte", npr / nde * gm)
+    B.let("S1", ac * (um[0] * uh[1] - uh[0] * um[1]))
+    B.let("S2", ac * (um[0] * uh[2] - uh[0] * um[2]))
+    B.let("S3", ac * (um[0] * uh[3] - uh[0] * um[3]))
+    B.let("S4", ac * nde * de * (half * ((nvx ** 2 - vx ** 2) + (nvy ** 2 - vy ** 2) + (nvz ** 2 - vz ** 2)) +
+                                 sp.Rational(3, 2) / gm * (nte - te)))
+    sig = ("PIP_HD void source_vjp(const double* umv, const double* uhv, double acv, double gm, "
+           "const double* lam, double* lum, double* luh, double* lac)")
+    pro = "".join(f"  const double um{k} = umv[{k}];\n" for k in range(8)) + \
+        "".join(f"  const double uh{k} = uhv[{k}];\n" for k in range(5)) + "  const double ac = acv;\n"
+    epi = "".join(f"  lum[{k}] += a_um{k};\n" for k in range(8)) + \
+        "".join(f"  luh[{k}] += a_uh{k};\n" for k in range(5)) + "  *lac += a_ac;\n"
+    seeds = [(f"S{k}", f"lam[{k}]") for k in range(1, 5)]
+    return g.emit(sig, pro, epi, seeds)
+
+
+def main():
+    parts = [
+        "// pip_adj_gen.h -- GENERATED by gen_adjoint.py; do not edit by hand.\n"
+        "// Vector-Jacobian products of the HLLC flux (HLL_rot.f90:615-687), the\n"
+        "// HLLD flux with HLLC-G fallback and GLM psi terms (HLL_rot.f90:691-910)\n"
+        "// and the collision source (PIP_rot.f90:1359-1368), at the branch taken\n"
+        "// by the forward point.  Each function accumulates into its outputs.\n"
+        "#pragma once\n#include \"pip_phys.h\"\n\nnamespace pip {\n\n",
+        gen_hllc(), "\n", gen_hlld(), "\n", gen_source(), "\n}  // namespace pip\n",
+    ]
+    path = os.path.join(HERE, "pip_adj_gen.h")
+    with open(path, "w") as f:
+        f.write("".join(parts))
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,31 +94,36 @@ __device__ __forceinline__ void stage_coeffs(int istep, int t_order, double dt, 
 }
 
 // Ghost images of interior index i along one direction of n cells padded by
-// 2 (C indices 2..n+1 interior): returns the count and fills dst[], with the
-// sign flag (mirror types flip the masked variables).
-__device__ __forceinline__ int ghost_images(int i, int npad, int tlo, int thi, int* dst, bool* flip) {
+// 2 (C indices 2..n+1 interior), as the inverse of the boundary_control
+// source map (Boundary_rot.f90:323-402).  Up to two images per side; image k
+// is packed as (dst + 1) in bits 8k..8k+6 and its mirror flag in bit 8k+7,
+// so the result stays in registers.
+__device__ __forceinline__ unsigned ghost_images(int i, int npad, int tlo, int thi) {
+  unsigned out = 0u;
   int k = 0;
-  // lower side
+  auto put = [&](int dst, bool flip) {
+    out |= ((unsigned)(dst + 1) | (flip ? 0x80u : 0u)) << (8 * k);
+    k++;
+  };
   if (tlo == 1) {
-    if (i == npad - 4) { dst[k] = 0; flip[k++] = false; }
-    if (i == npad - 3) { dst[k] = 1; flip[k++] = false; }
+    if (i == npad - 4) put(0, false);
+    if (i == npad - 3) put(1, false);
   } else if (tlo == 2) {
-    if (i == 3) { dst[k] = 0; flip[k++] = true; }
-    if (i == 2) { dst[k] = 1; flip[k++] = true; }
+    if (i == 3) put(0, true);
+    if (i == 2) put(1, true);
   } else if (tlo == 10) {
-    if (i == 2) { dst[k] = 0; flip[k++] = false; dst[k] = 1; flip[k++] = false; }
+    if (i == 2) { put(0, false); put(1, false); }
   }
-  // upper side
   if (thi == 1) {
-    if (i == 2) { dst[k] = npad - 2; flip[k++] = false; }
-    if (i == 3) { dst[k] = npad - 1; flip[k++] = false; }
+    if (i == 2) put(npad - 2, false);
+    if (i == 3) put(npad - 1, false);
   } else if (thi == 2) {
-    if (i == npad - 3) { dst[k] = npad - 2; flip[k++] = true; }
-    if (i == npad - 4) { dst[k] = npad - 1; flip[k++] = true; }
+    if (i == npad - 3) put(npad - 2, true);
+    if (i == npad - 4) put(npad - 1, true);
   } else if (thi == 10) {
-    if (i == npad - 3) { dst[k] = npad - 2; flip[k++] = false; dst[k] = npad - 1; flip[k++] = false; }
+    if (i == npad - 3) { put(npad - 2, false); put(npad - 1, false); }
   }
-  return k;
+  return out;
 }
 
 // Store one cell value set (NVT vars) at (ci,cj) and at all its ghost images.
@@ -127,29 +132,39 @@ __device__ __forceinline__ void store_cell(double* U, long long N, int ix, int j
                                            const double* u, const Bnd& b) {
 #pragma unroll
   for (int n = 0; n < NVT; n++) U[n * N + (long long)cj * ix + ci] = u[n];
-  int xd[4], yd[4];
-  bool xf[4], yf[4];
-  int nxi = ghost_images(ci, ix, b.type[0], b.type[1], xd, xf);
-  int nyi = 0;
-  if (NDIM == 2) nyi = ghost_images(cj, jx, b.type[2], b.type[3], yd, yf);
-  if (nxi == 0 && nyi == 0) return;
-  // x images in the own row
-  for (int a = 0; a < nxi; a++) {
-    unsigned m = xf[a] ? b.sym[(xd[a] < 2) ? 0 : 1] : 0u;
-    for (int n = 0; n < NVT; n++)
-      U[n * N + (long long)cj * ix + xd[a]] = ((m >> n) & 1u) ? -u[n] : u[n];
-  }
-  if (NDIM == 2) {
-    for (int c = 0; c < nyi; c++) {
-      unsigned my = yf[c] ? b.sym[(yd[c] < 2) ? 2 : 3] : 0u;
-      for (int n = 0; n < NVT; n++)
-        U[n * N + (long long)yd[c] * ix + ci] = ((my >> n) & 1u) ? -u[n] : u[n];
-      for (int a = 0; a < nxi; a++) {
-        unsigned mx = xf[a] ? b.sym[(xd[a] < 2) ? 0 : 1] : 0u;
-        unsigned m = mx ^ my;
-        for (int n = 0; n < NVT; n++)
-          U[n * N + (long long)yd[c] * ix + xd[a]] = ((m >> n) & 1u) ? -u[n] : u[n];
+  unsigned gx = ghost_images(ci, ix, b.type[0], b.type[1]);
+  unsigned gy = (NDIM == 2) ? ghost_images(cj, jx, b.type[2], b.type[3]) : 0u;
+  if (gx == 0u && gy == 0u) return;
+  // column images: the own column, then each x image
+  for (int a = -1; a < 4; a++) {
+    int xi;
+    unsigned mx;
+    if (a < 0) {
+      xi = ci;
+      mx = 0u;
+    } else {
+      unsigned e = (gx >> (8 * a)) & 0xffu;
+      if (e == 0u) break;
+      xi = (int)(e & 0x7fu) - 1;
+      mx = (e & 0x80u) ? b.sym[(xi < 2) ? 0 : 1] : 0u;
+    }
+    // row images: the own row (only for x images), then each y image
+    for (int c = -1; c < 4; c++) {
+      int yj;
+      unsigned my;
+      if (c < 0) {
+        if (a < 0) continue;
+        yj = cj;
+        my = 0u;
+      } else {
+        unsigned e = (gy >> (8 * c)) & 0xffu;
+        if (e == 0u) break;
+        yj = (int)(e & 0x7fu) - 1;
+        my = (e & 0x80u) ? b.sym[(yj < 2) ? 2 : 3] : 0u;
       }
+      unsigned m = mx ^ my;
+      for (int n = 0; n < NVT; n++)
+        U[n * N + (long long)yj * ix + xi] = ((m >> n) & 1u) ? -u[n] : u[n];
     }
   }
 }
@@ -227,6 +242,7 @@ __device__ __forceinline__ void block_reduce_store(RoundOut r, int nanflag, bool
 template <int NDIM, int NV, int TX, int TY>
 struct TileGeom {
   static constexpr int NVT = NV + 5;
+  static constexpr int NCELL = TX * TY;
   static constexpr int HX = TX + 4;
   static constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
   static constexpr int NFX = (TX + 1) * TY;
@@ -236,15 +252,16 @@ struct TileGeom {
 };
 
 // One SSP-RK stage on the tile of interior cells [i0,i0+TX) x [j0,j0+TY)
-// (interior coordinates), executed by all TX*TY threads of the workgroup.
+// (interior coordinates), executed by all NT threads of the workgroup.
+// NT >= TX*TY; shapes with (TX+1)*TY <= NT and TX*(TY+1) <= NT compute every
+// face phase in a single pass of the workgroup (e.g. 16x15 on 256 threads).
 // Returns whether the calling thread owns a cell; r / nanflag receive the
 // cell's dt terms when epi >= 1.
-template <int NDIM, int NV, bool IR, int TX, int TY>
+template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
 __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double cmax, int epi, int i0,
                                            int j0, double* smem, RoundOut& r, int& nanflag) {
   using G = TileGeom<NDIM, NV, TX, TY>;
   constexpr int NVT = G::NVT;
-  constexpr int NT = TX * TY;
   constexpr int HX = G::HX;
   constexpr int HY = G::HY;
   constexpr int OY = (NDIM == 2) ? 2 : 0;
@@ -315,7 +332,7 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
   // ---- own cell: sources and x update
   const int ti = tid % TX, tj = tid / TX;
   const int ci = i0 + ti + 2, cj = (NDIM == 2) ? j0 + tj + 2 : 0;
-  const bool valid = (i0 + ti < a.nx) && (j0 + tj < a.ny);
+  const bool valid = (tid < TX * TY) && (i0 + ti < a.nx) && (j0 + tj < a.ny);
   double un[NVT];
   double Sm[5], Sh[5];
   double ion_pot = 0.0;
@@ -428,8 +445,8 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 // ------------------------------------------------------------ stage kernel
 // One workgroup per tile.  The look-ahead round of the fused step (epi=2)
 // is skipped on the loop's last step (sc->last), as main.f90 ends there.
-template <int NDIM, int NV, bool IR, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
+template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   extern __shared__ double smem[];
   DevScalars* sc = a.sc;
   if (!sc->active) return;
@@ -442,10 +459,10 @@ __global__ void __launch_bounds__(TX * TY) k_stage(StageArgs a) {
   const int j0 = (bid / ntx) * TY;
   RoundOut r;
   int nanflag;
-  bool valid = stage_tile<NDIM, NV, IR, TX, TY>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
+  bool valid = stage_tile<NDIM, NV, IR, TX, TY, NT>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
   if (a.epi >= 1) {
     __syncthreads();
-    block_reduce_store<TX * TY>(r, nanflag, valid, sc, smem);
+    block_reduce_store<NT>(r, nanflag, valid, sc, smem);
   }
 }
 
@@ -722,7 +739,8 @@ template <int TX>
 __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
   extern __shared__ double smem[];
   __shared__ int s_member;
-  using G = TileGeom<1, 8, TX, 1>;
+  constexpr int TC = TX - 1;  // cells per tile: TC+1 faces = one pass of TX threads
+  using G = TileGeom<1, 8, TC, 1>;
   double* lds_red = smem + G::LDS_DOUBLES;  // 3 * (TX/64) doubles past the tile
   for (;;) {
     __syncthreads();
@@ -783,8 +801,8 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
       a.U0 = U;
       a.Uout = U1;
       a.epi = 0;
-      for (int i0 = 0; i0 < M.nx; i0 += TX) {
-        stage_tile<1, 8, false, TX, 1>(a, dt, 0.0, 0, i0, 0, smem, r, nanflag);
+      for (int i0 = 0; i0 < M.nx; i0 += TC) {
+        stage_tile<1, 8, false, TC, 1, TX>(a, dt, 0.0, 0, i0, 0, smem, r, nanflag);
         __syncthreads();
       }
       __threadfence_block();
@@ -796,8 +814,8 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
       const int epi = last ? 1 : 2;
       RoundOut acc = {1.0e300, 0.0, 0.0, 0.0};
       int fl = 0;
-      for (int i0 = 0; i0 < M.nx; i0 += TX) {
-        bool v = stage_tile<1, 8, false, TX, 1>(a, dt, 0.0, epi, i0, 0, smem, r, nanflag);
+      for (int i0 = 0; i0 < M.nx; i0 += TC) {
+        bool v = stage_tile<1, 8, false, TC, 1, TX>(a, dt, 0.0, epi, i0, 0, smem, r, nanflag);
         if (v) {
           acc.dtc = dmin(acc.dtc, r.dtc);
           acc.rhoac = dmax(acc.rhoac, r.rhoac);

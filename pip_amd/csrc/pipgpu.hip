@@ -46,7 +46,7 @@ struct pip_ctx {
   double* ion_pot = nullptr;
   DevScalars* sc = nullptr;
   hipStream_t stream = nullptr;
-  int tx = 32, ty = 8;
+  int tx = 16, ty = 15;
   Phys ph;
   IRConst irc;
   Bnd bnd;
@@ -142,24 +142,18 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   return a;
 }
 
-template <int NDIM, int NV, bool IR, int TX, int TY>
+template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
 static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
-  constexpr int NVT = NV + 5;
-  constexpr int HX = TX + 4;
-  constexpr int HY = (NDIM == 2) ? TY + 4 : 1;
-  constexpr int NFX = (TX + 1) * TY;
-  constexpr int NFY = (NDIM == 2) ? TX * (TY + 1) : 0;
-  constexpr int NF = (NFX > NFY) ? NFX : NFY;
-  constexpr size_t lds = sizeof(double) * (size_t)(NVT * HY * HX + NVT * NF);
+  constexpr size_t lds = sizeof(double) * (size_t)TileGeom<NDIM, NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage<NDIM, NV, IR, TX, TY>;
+  auto kern = k_stage<NDIM, NV, IR, TX, TY, NT>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
   }
   int ntx = (c->nx + TX - 1) / TX;
   int nty = (NDIM == 2) ? (c->ny + TY - 1) / TY : 1;
-  hipLaunchKernelGGL(kern, dim3(ntx * nty), dim3(TX * TY), lds, c->stream, a);
+  hipLaunchKernelGGL(kern, dim3(ntx * nty), dim3(NT), lds, c->stream, a);
   HIPCHK(hipGetLastError());
   return PIP_OK;
 }
@@ -167,13 +161,15 @@ static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
 template <int NDIM, int NV, bool IR>
 static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   if constexpr (NDIM == 1) {
-    return launch_stage_t<1, NV, IR, 256, 1>(c, a);
+    return launch_stage_t<1, NV, IR, 255, 1, 256>(c, a);
   } else {
-    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, 32, 8>(c, a);
-    if (c->tx == 16 && c->ty == 16) return launch_stage_t<2, NV, IR, 16, 16>(c, a);
-    if (c->tx == 64 && c->ty == 4) return launch_stage_t<2, NV, IR, 64, 4>(c, a);
-    if (c->tx == 32 && c->ty == 4) return launch_stage_t<2, NV, IR, 32, 4>(c, a);
-    if (c->tx == 16 && c->ty == 8) return launch_stage_t<2, NV, IR, 16, 8>(c, a);
+    // tile shapes (cells) and workgroup sizes; the single-pass shapes keep
+    // every face phase within one pass of the workgroup
+    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, 16, 15, 256>(c, a);
+    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, 32, 7, 256>(c, a);
+    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, 32, 15, 512>(c, a);
+    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, 32, 8, 256>(c, a);
+    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, 16, 7, 128>(c, a);
     return fail(PIP_ERR_ARG, "unsupported tile shape");
   }
 }
@@ -567,7 +563,7 @@ int pip_get_stream(pip_ctx* c, void** stream) {
 
 int pip_set_tile(pip_ctx* c, int tx, int ty) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
-  const int ok[][2] = {{32, 8}, {16, 16}, {64, 4}, {32, 4}, {16, 8}};
+  const int ok[][2] = {{16, 15}, {32, 7}, {32, 15}, {32, 8}, {16, 7}};
   for (auto& t : ok)
     if (t[0] == tx && t[1] == ty) {
       c->tx = tx;
@@ -644,7 +640,7 @@ struct pip_ens {
 
 template <int TX>
 static int ens_launch(pip_ens* e) {
-  using G = TileGeom<1, 8, TX, 1>;
+  using G = TileGeom<1, 8, TX - 1, 1>;
   size_t lds = sizeof(double) * (size_t)(G::LDS_DOUBLES + 3 * (TX / 64));
   static bool attr = false;
   auto kern = k_ensemble<TX>;
