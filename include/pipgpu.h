@@ -104,6 +104,36 @@ int pip_set_tile(pip_ctx *ctx, int tx, int ty);
  * measured with hipEvents on the context's stream. */
 int pip_time_steps(pip_ctx *ctx, int nsteps, double *ms_total, double *ms_stage1, double *ms_stage2);
 
+/* ---- discrete adjoint (no counterpart in the reference: PIP has no
+ * derivatives).  Supports the reverse pass of a torch.autograd.Function:
+ * record a run's dt/cmax sequence, replay it with frozen dt from a
+ * checkpoint while saving the stage inputs, and pull an adjoint back
+ * through one step.  All pointers are device pointers with the nvar_m + 5
+ * plane layout of pip_device_state; calls are stream-ordered on the
+ * context's stream (pip_sync before reading results from another stream).
+ * Adjoint steps cover ndim = 2, flag_divb = 1, flag_IR = 0, SSP-RK2 and
+ * periodic or mirror boundaries. */
+/* device -> device copies of the state and of the per-cell alpha_c */
+int pip_set_state_device(pip_ctx *ctx, const double *U);
+int pip_get_state_device(pip_ctx *ctx, double *U);
+int pip_set_ac_device(pip_ctx *ctx, const double *ac);
+/* n fused steps of the time loop (main.f90:31-68, no end time) recording
+ * hist[2k] = dt and hist[2k+1] = cmax of step k (device array, 2n doubles) */
+int pip_record_steps(pip_ctx *ctx, int n, double *hist);
+/* n steps with dt/cmax taken from hist (frozen-dt replay of a recorded run,
+ * bitwise equal to it from the same state); when Un / U1 are non-NULL the
+ * stage-1 input U_n and the stage-2 input U1 of step k are saved at plane
+ * offset k*(nvar_m+5) of each array */
+int pip_replay_steps(pip_ctx *ctx, int n, const double *hist, double *Un, double *U1);
+/* adjoint seed: add the ghost-cell entries of lam onto the interior cells
+ * they image (boundary sign maps), then zero the ghosts */
+int pip_adj_seed(pip_ctx *ctx, double *lam);
+/* one step backwards: on entry lam = dJ/dU_{n+1} (interior cells), on exit
+ * lam = dJ/dU_n; lam_ac (may be NULL) accumulates dJ/dalpha_c.  Un, U1 are
+ * the saved stage inputs of step n and dtc = &hist[2n]. */
+int pip_adj_step(pip_ctx *ctx, const double *Un, const double *U1, const double *dtc, double *lam,
+                 double *lam_ac);
+
 /* ---- batched ensemble (no counterpart in the reference: PIP runs one
  * domain per process).  Independent 1-D two-fluid runs of different sizes,
  * each with its own dt/time/step count, integrated in ONE persistent launch

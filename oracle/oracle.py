@@ -54,6 +54,8 @@ def lib():
         L.orc_get_rates.argtypes = [ctypes.c_void_p, P, P, P, P]
         L.orc_set_rates.argtypes = [ctypes.c_void_p, P, P]
         L.orc_get_scalars.argtypes = [ctypes.c_void_p, P]
+        L.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_set_stage_save.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         for name in ("orc_pipbnd", "orc_set_ir", "orc_run_solver", "orc_prologue"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.orc_set_coefficients.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -151,6 +153,22 @@ class Oracle:
         o = np.empty(4)
         lib().orc_get_scalars(self.h, _p(o))
         return dict(dt=o[0], time=o[1], cmax=o[2], nt=int(o[3]))
+
+    def set_scalars(self, dt: float, cmax: float):
+        """Test hook: dt and GLM speed cmax for the next run_solver."""
+        lib().orc_set_scalars(self.h, float(dt), float(cmax))
+
+    def run_solver_stage1(self):
+        """run_solver, also returning the stage-1 result (nvar_m+5, jx, ix)
+        as (U_m planes, U_h planes)."""
+        c = self.cfg
+        buf = np.empty((c.nvar_m + 5, c.jx, c.ix))
+        lib().orc_set_stage_save(self.h, buf.ctypes.data)
+        try:
+            lib().orc_run_solver(self.h)
+        finally:
+            lib().orc_set_stage_save(self.h, None)
+        return buf
 
     # reference entry points ---------------------------------------------
     def step(self) -> int:

@@ -36,6 +36,7 @@ struct orc_state {
     double dt, time, cmax;
     long nt;
     int sym_m[4][9], sym_h[4][5];
+    double *stage_save;    /* optional copy of the stage-1 result (test hook) */
     /* scratch */
     double *V, *VL, *VR, *Fm, *Fh, *Sm, *Sh, *Um0, *Uh0, *tmp;
 };
@@ -177,6 +178,13 @@ void orc_set_fixed_rates(orc_state *s, int on) { s->fixed_rates = on; }
 void orc_get_scalars(const orc_state *s, double *o) {
     o[0] = s->dt; o[1] = s->time; o[2] = s->cmax; o[3] = (double)s->nt;
 }
+
+void orc_set_scalars(orc_state *s, double dt, double cmax) {
+    s->dt = dt;
+    s->cmax = cmax;
+}
+
+void orc_set_stage_save(orc_state *s, double *buf) { s->stage_save = buf; }
 
 /* -------------------------------------------------------------- limiter */
 /* minmod_func: Util_rot.f90:59-76 */
@@ -929,6 +937,10 @@ void orc_run_solver(orc_state *s) {
             for (int k = 0; k < N; k++) s->Um[8 * N + k] = s->Um[8 * N + k] * f;
         }
         orc_pipbnd(s); /* PIPbnd(U_h,U_m,istep) :70 */
+        if (istep == 1 && s->stage_save) {
+            memcpy(s->stage_save, s->Um, sizeof(double) * N * nv);
+            memcpy(s->stage_save + (size_t)N * nv, s->Uh, sizeof(double) * N * 5);
+        }
     }
     orc_pipbnd(s); /* post_step is a no-op for HLL; PIPbnd(.,0) :77 */
 }

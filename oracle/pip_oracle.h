@@ -48,6 +48,10 @@ void orc_get_rates(const orc_state *s, double *gm_rec, double *gm_ion,
                    double *arb_heat, double *ion_pot);
 void orc_set_rates(orc_state *s, const double *gm_rec, const double *gm_ion);
 void orc_get_scalars(const orc_state *s, double *out4); /* dt,time,cmax,nt */
+/* test hooks: set dt and the GLM speed cmax used by orc_run_solver; copy
+ * the stage-1 result (U_m planes then U_h planes) into buf (NULL: off) */
+void orc_set_scalars(orc_state *s, double dt, double cmax);
+void orc_set_stage_save(orc_state *s, double *buf);
 
 /* reference entry points (Fortran names) */
 void orc_pipbnd(orc_state *s);                 /* Boundary_rot.f90:34-52   */

@@ -1,0 +1,186 @@
+"""Differentiable two-fluid runs: a torch.autograd.Function over the adjoint
+entry points of libpipgpu (pip_record_steps, pip_replay_steps, pip_adj_seed,
+pip_adj_step).
+
+``TwoFluidRun(cfg, nsteps, ckpt_every=20)`` integrates ``nsteps`` steps of
+the time loop (main.f90:31-68: prologue, then fused set_IR/cfl/run_solver
+steps) from a device state ``U0`` (nvar_m + 5, jx, ix) with a per-cell
+collision coefficient field ``ac`` (jx, ix).  ``run(U0, ac)`` returns the
+final state and is differentiable with respect to both inputs:
+
+  forward   records dt/cmax of every step on the device and keeps the state
+            at every ``ckpt_every``-th step (step checkpointing);
+  backward  for each segment, newest first: restore its checkpoint, replay
+            the segment with the recorded dt (bitwise the forward run)
+            saving the stage inputs, then pull the adjoint back through
+            its steps (hand-written reverse-mode kernels: stage update,
+            HLLD/HLLC fluxes, minmod reconstruction, collision source).
+
+The gradient is that of the discrete map with the recorded time-step
+sequence held fixed (dt and the GLM speed cmax are treated as constants);
+``replay(U0, ac)`` evaluates exactly that map, so central differences of
+``replay`` check the gradient.  The library is the only compute path: with
+no built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .config import PIPConfig
+from ._lib import check, lib
+from .solver import PIPSolver
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+class TwoFluidRun:
+    def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
+                 device: str = "cuda:0"):
+        if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
+            raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
+        if not cfg.ac_field:
+            raise ValueError("the adjoint run needs a per-cell ac field (cfg.ac_field)")
+        if nsteps < 0 or ckpt_every < 1:
+            raise ValueError("nsteps >= 0 and ckpt_every >= 1")
+        self.cfg = cfg
+        self.nsteps = int(nsteps)
+        self.ckpt_every = int(ckpt_every)
+        self.device = torch.device(device)
+        self.nvt = cfg.nvar_m + 5
+        z = np.zeros((cfg.jx, cfg.ix))
+        self.solver = PIPSolver(cfg, np.zeros((5, cfg.jx, cfg.ix)), np.zeros((cfg.nvar_m, cfg.jx, cfg.ix)),
+                                z, prologue=False, tile=tile)
+        self.h = self.solver.h
+        self.hist = torch.zeros(2 * max(self.nsteps, 1), dtype=torch.float64, device=self.device)
+        self._seg_Un = None
+        self._seg_U1 = None
+
+    # ------------------------------------------------------------ helpers
+    def _segments(self):
+        out = []
+        k = 0
+        while k < self.nsteps:
+            m = min(self.ckpt_every, self.nsteps - k)
+            out.append((k, m))
+            k += m
+        return out
+
+    def _torch_done(self):
+        torch.cuda.current_stream(self.device).synchronize()
+
+    def _lib_done(self):
+        self.solver.sync()
+
+    def _start(self, U0: torch.Tensor, ac: torch.Tensor):
+        """Load (U0, ac), run the prologue (Initial_rot.f90:140-160) and the
+        first loop-top set_IR/cfl round (main.f90:32-36)."""
+        self._check_inputs(U0, ac)
+        self._torch_done()
+        L = lib()
+        check(L.pip_set_state_device(self.h, _ptr(U0)))
+        check(L.pip_set_ac_device(self.h, _ptr(ac)))
+        dt = ctypes.c_double()
+        check(L.pip_prologue(self.h, ctypes.byref(dt)))
+        check(L.pip_record_steps(self.h, 0, None))
+
+    def _check_inputs(self, U0, ac):
+        c = self.cfg
+        if tuple(U0.shape) != (self.nvt, c.jx, c.ix) or U0.dtype != torch.float64 or not U0.is_contiguous():
+            raise ValueError(f"U0 must be a contiguous float64 tensor of shape {(self.nvt, c.jx, c.ix)}")
+        if tuple(ac.shape) != (c.jx, c.ix) or ac.dtype != torch.float64 or not ac.is_contiguous():
+            raise ValueError(f"ac must be a contiguous float64 tensor of shape {(c.jx, c.ix)}")
+        if U0.device != self.device or ac.device != self.device:
+            raise ValueError(f"inputs must live on {self.device}")
+
+    def _empty_state(self, n=None):
+        c = self.cfg
+        shape = (self.nvt, c.jx, c.ix) if n is None else (n, self.nvt, c.jx, c.ix)
+        return torch.empty(shape, dtype=torch.float64, device=self.device)
+
+    # ------------------------------------------------------------ passes
+    def forward_record(self, U0: torch.Tensor, ac: torch.Tensor):
+        """Forward run recording dt/cmax; returns (U_T, checkpoints)."""
+        self._start(U0, ac)
+        L = lib()
+        ckpts = []
+        for k0, m in self._segments():
+            ck = self._empty_state()
+            check(L.pip_get_state_device(self.h, _ptr(ck)))
+            ckpts.append(ck)
+            check(L.pip_record_steps(self.h, m, _ptr(self.hist) + 16 * k0))
+        out = self._empty_state()
+        check(L.pip_get_state_device(self.h, _ptr(out)))
+        self._lib_done()
+        return out, ckpts
+
+    def replay(self, U0: torch.Tensor, ac: torch.Tensor) -> torch.Tensor:
+        """The same run with the recorded dt/cmax sequence held fixed (the map
+        whose derivative backward() returns)."""
+        self._start(U0, ac)
+        check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(self.hist), None, None))
+        out = self._empty_state()
+        check(lib().pip_get_state_device(self.h, _ptr(out)))
+        self._lib_done()
+        return out
+
+    def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor):
+        """dJ/dU0, dJ/dac from dJ/dU_T by checkpointed reverse sweeps."""
+        L = lib()
+        lam = gU.detach().to(torch.float64).contiguous().clone()
+        lam_ac = torch.zeros_like(ac)
+        segs = self._segments()
+        mmax = max((m for _, m in segs), default=0)
+        if mmax and (self._seg_Un is None or self._seg_Un.shape[0] < mmax):
+            self._seg_Un = self._empty_state(mmax)
+            self._seg_U1 = self._empty_state(mmax)
+        self._torch_done()
+        check(L.pip_set_ac_device(self.h, _ptr(ac)))
+        check(L.pip_adj_seed(self.h, _ptr(lam)))
+        sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
+        for (k0, m), ck in zip(reversed(segs), reversed(ckpts)):
+            check(L.pip_set_state_device(self.h, _ptr(ck)))
+            check(L.pip_replay_steps(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
+                                     _ptr(self._seg_U1)))
+            for k in reversed(range(m)):
+                check(L.pip_adj_step(self.h, _ptr(self._seg_Un) + sb * k, _ptr(self._seg_U1) + sb * k,
+                                     _ptr(self.hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
+        self._lib_done()
+        return lam, lam_ac
+
+    def run(self, U0: torch.Tensor, ac: torch.Tensor) -> torch.Tensor:
+        """Differentiable run: U_T = run(U0, ac)."""
+        return _Evolve.apply(U0, ac, self)
+
+    def dt_history(self) -> np.ndarray:
+        self._lib_done()
+        return self.hist[: 2 * self.nsteps].view(-1, 2)[:, 0].cpu().numpy()
+
+    def close(self):
+        self.solver.close()
+
+
+class _Evolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, ac, runner: TwoFluidRun):
+        U0c = U0.detach().contiguous()
+        acc = ac.detach().contiguous()
+        out, ckpts = runner.forward_record(U0c, acc)
+        ctx.runner = runner
+        ctx.ckpts = ckpts
+        ctx.save_for_backward(acc)
+        return out
+
+    @staticmethod
+    def backward(ctx, gU):
+        (acc,) = ctx.saved_tensors
+        lam, lam_ac = ctx.runner.backward(gU, ctx.ckpts, acc)
+        return lam, lam_ac, None
+
+
+def evolve(runner: TwoFluidRun, U0: torch.Tensor, ac: torch.Tensor) -> torch.Tensor:
+    return runner.run(U0, ac)

@@ -1,7 +1,13 @@
 // hostcheck.cpp -- host build of the per-face/per-cell device functions
-// (pip_phys.h, pip_adj_gen.h) for CPU tests: forward fluxes and their
-// vector-Jacobian products.  Test infrastructure; not used by libpipgpu.
+// (pip_phys.h, pip_adj_gen.h, pip_adj_point.h) for CPU tests: forward
+// fluxes, reconstruction and conversions, and their vector-Jacobian
+// products.  Test infrastructure; not used by libpipgpu.
+#include <vector>
+
 #include "pip_adj_gen.h"
+#include "pip_adj_items.h"
+#include "pip_adj_point.h"
+#include "pip_bnd.h"
 #include "pip_phys.h"
 
 using namespace pip;
@@ -35,6 +41,106 @@ void hc_source(const double* um, const double* uh, double ac, double gm, double*
 void hc_source_vjp(const double* um, const double* uh, double ac, double gm, const double* lam, double* lum,
                    double* luh, double* lac) {
   source_vjp(um, uh, ac, gm, lam, lum, luh, lac);
+}
+double hc_minmod(double a, double b, double c) { return minmod(a, b, c); }
+void hc_recon_grad(double v0, double v1, double v2, int right, double* g) {
+  if (right) recon_right_grad(v0, v1, v2, g[0], g[1], g[2]);
+  else recon_left_grad(v0, v1, v2, g[0], g[1], g[2]);
+}
+static Phys hc_phys(double gm, double ro_lim, double pr_lim) {
+  Phys p{};
+  p.gm = gm;
+  p.ro_lim = ro_lim;
+  p.pr_lim = pr_lim;
+  return p;
+}
+void hc_u2v9(const double* u, double gm, double ro_lim, double pr_lim, double* v) {
+  u2v_mhd<9>(u, v, hc_phys(gm, ro_lim, pr_lim));
+}
+void hc_u2v9_vjp(const double* u, const double* lv, double gm, double ro_lim, double pr_lim, double* lu) {
+  u2v_vjp<9>(u, lv, hc_phys(gm, ro_lim, pr_lim), lu);
+}
+void hc_u2v_hd(const double* u, double gm, double ro_lim, double pr_lim, double* v) {
+  u2v_hd(u, v, hc_phys(gm, ro_lim, pr_lim));
+}
+void hc_u2v_hd_vjp(const double* u, const double* lv, double gm, double ro_lim, double pr_lim, double* lu) {
+  u2v_vjp_hd(u, lv, hc_phys(gm, ro_lim, pr_lim), lu);
+}
+int hc_src_index(int g, int npad, int tlo, int thi, int* flip) {
+  bool f;
+  int s = src_index(g, npad, tlo, thi, f);
+  *flip = f ? 1 : 0;
+  return s;
+}
+// ghost images of interior cell (ci, cj): out[3k..3k+2] = (xi, yj, mask)
+int hc_images(int ci, int cj, int ix, int jx, const int* type, const unsigned* sym, int* out) {
+  Bnd b;
+  for (int s = 0; s < 4; s++) {
+    b.type[s] = type[s];
+    b.sym[s] = sym[s];
+  }
+  int n = 0;
+  for_each_image(ci, cj, ix, jx, b, [&](int xi, int yj, unsigned m) {
+    out[3 * n] = xi;
+    out[3 * n + 1] = yj;
+    out[3 * n + 2] = (int)m;
+    n++;
+  });
+  return n;
+}
+// One adjoint step on the host (the same work items as the HIP kernels, run
+// in sequence): p = {gm, ro_lim, pr_lim, dx, dy, col, safety, db_clean, dt,
+// cmax}; flag_bnd[4]; on entry lam = dJ/dU_{n+1}, on exit dJ/dU_n.
+void hc_adj_step(int ix, int jx, const double* p, const int* flag_bnd, const double* Un, const double* U1,
+                 const double* ac, double* lam, double* lam_ac) {
+  const int NVT = ADJ_NVT;
+  const long long N = (long long)ix * jx;
+  Phys ph{};
+  ph.gm = p[0];
+  ph.ro_lim = p[1];
+  ph.pr_lim = p[2];
+  ph.safety = p[6];
+  std::vector<double> V(NVT * N), P(8 * NVT * N, 0.0), lamB(NVT * N, 0.0), lamC(NVT * N, 0.0);
+  const double dtc[2] = {p[8], p[9]};
+  const Bnd bnd = make_bnd(flag_bnd, 2, ADJ_NV, 1);
+  for (long long k = 0; k < N; k++) adj_fold_seed_item(lam, NVT, ix, jx, N, bnd, k);
+  for (long long k = 0; k < N; k++) zero_ghosts_item(lam, NVT, ix, jx, N, k);
+  for (int stage = 2; stage >= 1; stage--) {
+    AdjArgs a;
+    a.Uin = (stage == 2) ? U1 : Un;
+    a.lam_out = (stage == 2) ? lam : lamB.data();
+    a.lam_in = (stage == 2) ? lamB.data() : lamC.data();
+    a.lam_U0 = lamC.data();
+    a.lam_ac = lam_ac;
+    a.ac = ac;
+    a.V = V.data();
+    a.P = P.data();
+    a.dtc = dtc;
+    a.ix = ix;
+    a.jx = jx;
+    a.nx = ix - 4;
+    a.ny = jx - 4;
+    a.N = N;
+    a.istep = stage;
+    a.t_order = 2;
+    a.dx = p[3];
+    a.dy = p[4];
+    a.col = p[5];
+    a.safety = p[6];
+    a.db_clean = p[7];
+    a.ph = ph;
+    a.bnd = bnd;
+    for (long long k = 0; k < N; k++) adj_cell_item(a, k);
+    for (int dir = 0; dir < 2; dir++) {
+      const long long nf = (dir == 0) ? (long long)(a.nx + 1) * a.ny : (long long)a.nx * (a.ny + 1);
+      for (long long t = 0; t < nf; t++) {
+        adj_face_item<0>(a, dir, t);
+        adj_face_item<1>(a, dir, t);
+      }
+    }
+    for (long long k = 0; k < N; k++) adj_gather_item(a, k);
+  }
+  for (long long k = 0; k < NVT * N; k++) lam[k] = lamC[k];
 }
 double hc_exp(double x) { return pexp(x); }
 double hc_pow(double x, double y) { return ppow(x, y); }

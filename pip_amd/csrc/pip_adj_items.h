@@ -1,0 +1,283 @@
+// pip_adj_items.h -- per-cell and per-face work items of the discrete
+// adjoint stage (host and device): the HIP kernels of pip_adjoint.h run one
+// item per thread, the host checks loop over them.
+//
+// The adjoint covers one SSP-RK2 step of the 2-D two-fluid update
+// (nvar_m = 9 with Dedner psi, explicit collisions, no
+// ionisation/recombination), for gradients with respect to the per-cell
+// collision coefficient alpha_c and the initial state.
+//
+// Forward stage s (Solver_rot.f90:222-245, HLL_rot.f90:94-121):
+//   Uout_c = base_c - a (Fx_{c+1/2} - Fx_{c-1/2}) - b (Fy_{c+1/2} - Fy_{c-1/2})
+//            + dt_sub S(Uin_c, ac_c);   psi_out *= f  (Scheme_rot.f90:546-554)
+//   base = U0 (s = 1) or c0 U0 + c1 Uin (s = 2), F = H(recon(u2v(Uin))).
+// Reverse, with dt and cmax frozen at the values of the forward step and
+// every branch (minmod slope, HLLD region, floors) taken as at the forward
+// point; the cq2pv / get_Pr rewrites only re-round the state and pass
+// adjoints through unchanged:
+//   mu = lam_out on interior cells (psi component times f)
+//   lam_U0 += c0 mu,  lam_in += c1 mu (s = 2)
+//   lam_in += dt_sub (dS/dUin)^T mu,  lam_ac += dt_sub (dS/dac)^T mu
+//   per face: lamF = a (mu_R - mu_L) -> H^T -> recon^T: four per-cell
+//             contributions written to their own planes (no atomics, so the
+//             result is deterministic)
+//   per cell: gather those planes at the cell and at its ghost images
+//             (periodic / mirror sign maps), then u2v^T into lam_in.
+#pragma once
+
+#include "pip_adj_gen.h"
+#include "pip_adj_point.h"
+#include "pip_bnd.h"
+#include "pip_phys.h"
+
+namespace pip {
+
+struct AdjArgs {
+  const double* Uin;      // stage input (nvt planes, ghosts filled)
+  const double* lam_out;  // adjoint of the stage output
+  double* lam_in;         // += adjoint of Uin
+  double* lam_U0;         // += adjoint of U0 (may alias lam_in for s = 1)
+  double* lam_ac;         // += adjoint of alpha_c (nullable)
+  const double* ac;       // per-cell alpha_c (nullable: constant col)
+  double* V;              // nvt planes: primitives of Uin (all cells)
+  double* P;              // 2 * 4 * nvt planes: per-face contributions
+  const double* dtc;      // device: {dt, cmax} of the forward step
+  int ix, jx, nx, ny;
+  long long N;
+  int istep, t_order;
+  double dx, dy, col, safety, db_clean;
+  Phys ph;
+  Bnd bnd;
+};
+
+constexpr int ADJ_NV = 9;
+constexpr int ADJ_NVT = ADJ_NV + 5;
+
+PIP_HD void adj_coeffs(const AdjArgs& a, double& c0, double& c1, double& dt_sub,
+                                           double& psi_fac) {
+  const double dt = a.dtc[0];
+  stage_coeffs(a.istep, a.t_order, dt, c0, c1, dt_sub);
+  psi_fac = pexp(-dt_sub / dt * a.safety * a.db_clean);
+}
+
+PIP_HD bool adj_interior(const AdjArgs& a, int i, int j) {
+  return i >= 2 && i < a.ix - 2 && j >= 2 && j < a.jx - 2;
+}
+
+// adjoint of the stage output at an interior cell, psi scaled by the damping
+PIP_HD double adj_mu(const AdjArgs& a, int n, long long o, double psi_fac) {
+  const double m = a.lam_out[n * a.N + o];
+  return (n == 8) ? m * psi_fac : m;
+}
+
+// Per cell (all cells): primitives of Uin; interior cells also take the
+// SSP blend and the collision source.
+PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  if (k >= a.N) return;
+  const long long N = a.N;
+  double um[NV], uh[5];
+#pragma unroll
+  for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + k];
+#pragma unroll
+  for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + k];
+  {
+    double vm[NV], vh[5];
+    u2v_mhd<NV>(um, vm, a.ph);
+    u2v_hd(uh, vh, a.ph);
+#pragma unroll
+    for (int n = 0; n < NV; n++) a.V[n * N + k] = vm[n];
+#pragma unroll
+    for (int n = 0; n < 5; n++) a.V[(NV + n) * N + k] = vh[n];
+  }
+  const int i = (int)(k % a.ix), j = (int)(k / a.ix);
+  if (!adj_interior(a, i, j)) return;
+  double c0, c1, dt_sub, psi_fac;
+  adj_coeffs(a, c0, c1, dt_sub, psi_fac);
+  double mu[NVT];
+#pragma unroll
+  for (int n = 0; n < NVT; n++) mu[n] = adj_mu(a, n, k, psi_fac);
+  if (a.istep == 1) {
+#pragma unroll
+    for (int n = 0; n < NVT; n++) a.lam_U0[n * N + k] += mu[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NVT; n++) {
+      a.lam_U0[n * N + k] += c0 * mu[n];
+      a.lam_in[n * N + k] += c1 * mu[n];
+    }
+  }
+  // S_m[0..4] = +dS, S_h[0..4] = -dS (PIP_rot.f90:1366-1367)
+  double lam[5];
+#pragma unroll
+  for (int n = 0; n < 5; n++) lam[n] = dt_sub * (mu[n] - mu[NV + n]);
+  double lum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, luh[5] = {0, 0, 0, 0, 0}, lac = 0.0;
+  const double acv = a.ac ? a.ac[k] : a.col;
+  source_vjp(um, uh, acv, a.ph.gm, lam, lum, luh, &lac);
+#pragma unroll
+  for (int n = 0; n < 8; n++) a.lam_in[n * N + k] += lum[n];
+#pragma unroll
+  for (int n = 0; n < 5; n++) a.lam_in[(NV + n) * N + k] += luh[n];
+  if (a.lam_ac) a.lam_ac[k] += lac;
+}
+
+// Swap of the normal/tangential components for y faces (HLL_rot.f90:414-434
+// and :315-325); an involution.
+PIP_HD int adj_perm(int n, int dir) {
+  if (dir == 0) return n;
+  if (n == 1) return 2;
+  if (n == 2) return 1;
+  if (n == 5) return 6;
+  if (n == 6) return 5;
+  if (n == ADJ_NV + 1) return ADJ_NV + 2;
+  if (n == ADJ_NV + 2) return ADJ_NV + 1;
+  return n;
+}
+
+// One thread per face and fluid (FL 0: plasma, HLLD with GLM; FL 1:
+// neutrals, HLLC); R = the cell on the + side of the face.
+// dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
+// Writes P[(dir*4 + q) * NVT + n][R] = contribution to cell R + (q-2) st.
+template <int FL>
+PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  constexpr int NF = FL ? 5 : NV;   // variables of this fluid
+  constexpr int OFF = FL ? NV : 0;  // first plane of this fluid
+  const int wx = (dir == 0) ? a.nx + 1 : a.nx;
+  const int wy = (dir == 0) ? a.ny : a.ny + 1;
+  if (t >= (long long)wx * wy) return;
+  const int fi = (int)(t % wx) + 2, fj = (int)(t / wx) + 2;
+  const long long N = a.N;
+  const long long st = (dir == 0) ? 1 : a.ix;
+  const long long oR = (long long)fj * a.ix + fi;
+  const long long oL = oR - st;
+  double c0, c1, dt_sub, psi_fac;
+  adj_coeffs(a, c0, c1, dt_sub, psi_fac);
+  const bool intL = (dir == 0) ? (fi - 1 >= 2) : (fj - 1 >= 2);
+  const bool intR = (dir == 0) ? (fi < a.ix - 2) : (fj < a.jx - 2);
+  const double coef = dt_sub / ((dir == 0) ? a.dx : a.dy);
+
+  double vl[NF], vr[NF], lamF[NF];
+#pragma unroll
+  for (int n = 0; n < NF; n++) {
+    const int s = adj_perm(OFF + n, dir);
+    const double* p = a.V + s * N + oR;
+    const double v0 = p[-2 * st], v1 = p[-st], v2 = p[0], v3 = p[st];
+    vl[n] = v1 + 0.5 * minmod(v0, v1, v2);
+    vr[n] = v2 - 0.5 * minmod(v1, v2, v3);
+    const double mR = intR ? adj_mu(a, s, oR, psi_fac) : 0.0;
+    const double mL = intL ? adj_mu(a, s, oL, psi_fac) : 0.0;
+    lamF[n] = coef * (mR - mL);
+  }
+  // density floor of get_Side (HLL_rot.f90:60-61)
+  const bool flL = !(vl[0] > a.ph.ro_lim), flR = !(vr[0] > a.ph.ro_lim);
+  vl[0] = dmax(vl[0], a.ph.ro_lim);
+  vr[0] = dmax(vr[0], a.ph.ro_lim);
+  double lvl[NF], lvr[NF];
+#pragma unroll
+  for (int n = 0; n < NF; n++) { lvl[n] = 0.0; lvr[n] = 0.0; }
+  if (FL == 0) hlld9_flux_vjp(vl, vr, a.ph.gm, a.dtc[1], lamF, lvl, lvr);
+  else hllc_flux_vjp(vl, vr, a.ph.gm, lamF, lvl, lvr);
+  if (flL) lvl[0] = 0.0;
+  if (flR) lvr[0] = 0.0;
+  double* P = a.P + (long long)dir * 4 * NVT * N;
+#pragma unroll
+  for (int n = 0; n < NF; n++) {
+    const int s = adj_perm(OFF + n, dir);
+    const double* p = a.V + s * N + oR;
+    const double v0 = p[-2 * st], v1 = p[-st], v2 = p[0], v3 = p[st];
+    double gl0, gl1, gl2, gr0, gr1, gr2;
+    recon_left_grad(v0, v1, v2, gl0, gl1, gl2);
+    recon_right_grad(v1, v2, v3, gr0, gr1, gr2);
+    P[(0 * NVT + s) * N + oR] = lvl[n] * gl0;
+    P[(1 * NVT + s) * N + oR] = lvl[n] * gl1 + lvr[n] * gr0;
+    P[(2 * NVT + s) * N + oR] = lvl[n] * gl2 + lvr[n] * gr1;
+    P[(3 * NVT + s) * N + oR] = lvr[n] * gr2;
+  }
+}
+
+// Sum of the face contributions to the primitives of the cell at (i, j),
+// with the sign map `mask` of a ghost image applied.
+PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double* lv) {
+  constexpr int NVT = ADJ_NVT;
+  const long long N = a.N;
+  const long long o = (long long)j * a.ix + i;
+  const double* Px = a.P;
+  const double* Py = a.P + 4LL * NVT * N;
+  const bool rowx = (j >= 2 && j <= a.ny + 1);
+  const bool colx = (i >= 2 && i <= a.nx + 1);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int ri = i + 2 - q;  // x face with R at (ri, j)
+    if (rowx && ri >= 2 && ri <= a.nx + 2) {
+      const long long r = o + (2 - q);
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        const double v = Px[(q * NVT + n) * N + r];
+        lv[n] += ((mask >> n) & 1u) ? -v : v;
+      }
+    }
+    const int rj = j + 2 - q;  // y face with R at (i, rj)
+    if (colx && rj >= 2 && rj <= a.ny + 2) {
+      const long long r = o + (long long)(2 - q) * a.ix;
+#pragma unroll
+      for (int n = 0; n < NVT; n++) {
+        const double v = Py[(q * NVT + n) * N + r];
+        lv[n] += ((mask >> n) & 1u) ? -v : v;
+      }
+    }
+  }
+}
+
+// Per interior cell: gather the face contributions at the cell and at its
+// ghost images, then the transpose of u2v (HLL_rot.f90:1136-1186) with the
+// forward point's floors, into lam_in.
+PIP_HD void adj_gather_item(const AdjArgs& a, long long k) {
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  if (k >= a.N) return;
+  const int ci = (int)(k % a.ix), cj = (int)(k / a.ix);
+  if (!adj_interior(a, ci, cj)) return;
+  const long long N = a.N;
+  double lv[NVT];
+#pragma unroll
+  for (int n = 0; n < NVT; n++) lv[n] = 0.0;
+  adj_gather_at(a, ci, cj, 0u, lv);
+  for_each_image(ci, cj, a.ix, a.jx, a.bnd,
+                 [&](int xi, int yj, unsigned m) { adj_gather_at(a, xi, yj, m, lv); });
+  double u[NVT], lu[NVT];
+#pragma unroll
+  for (int n = 0; n < NVT; n++) u[n] = a.Uin[n * N + k];
+#pragma unroll
+  for (int n = 0; n < NVT; n++) lu[n] = 0.0;
+  u2v_vjp<NV>(u, lv, a.ph, lu);
+  u2v_vjp_hd(u + NV, lv + NV, a.ph, lu + NV);
+#pragma unroll
+  for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] += lu[n];
+}
+
+// Adjoint seed on the ghost cells of a state: add each ghost's value (with
+// its boundary sign map) onto the interior cell it images, so that the
+// adjoint kernels only read interior cells.
+PIP_HD void adj_fold_seed_item(double* lam, int nvt, int ix, int jx, long long N, const Bnd& bnd, long long k) {
+  if (k >= N) return;
+  const int ci = (int)(k % ix), cj = (int)(k / ix);
+  if (!(ci >= 2 && ci < ix - 2 && cj >= 2 && cj < jx - 2)) return;
+  for_each_image(ci, cj, ix, jx, bnd, [&](int xi, int yj, unsigned m) {
+    const long long g = (long long)yj * ix + xi;
+    for (int n = 0; n < nvt; n++) {
+      const double v = lam[n * N + g];
+      lam[n * N + k] += ((m >> n) & 1u) ? -v : v;
+    }
+  });
+}
+
+// Zero the ghost cells of an nvt-plane state (the adjoint carries values on
+// interior cells only).
+PIP_HD void zero_ghosts_item(double* x, int nvt, int ix, int jx, long long N, long long k) {
+  if (k >= N) return;
+  const int i = (int)(k % ix), j = (int)(k / ix);
+  if (i >= 2 && i < ix - 2 && j >= 2 && j < jx - 2) return;
+  for (int n = 0; n < nvt; n++) x[n * N + k] = 0.0;
+}
+
+}  // namespace pip

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pip_bnd.h"
 #include "pip_phys.h"
 
 namespace pip {
@@ -44,11 +45,6 @@ struct DevScalars {
 
 // Boundary description: per side type (1 periodic, 2 mirror, 10 free) and
 // sign masks (bit n set -> variable n flips sign), Boundary_rot.f90:239-320.
-struct Bnd {
-  int type[4];
-  unsigned sym[4];  // bits over NVT variables (U_m planes then U_h planes)
-};
-
 struct StageArgs {
   const double* Uin;  // stage input (stencil)
   const double* U0;   // state at run_solver entry (pointwise; may alias Uout)
@@ -81,92 +77,15 @@ PIP_HD double as_f64(unsigned long long x) {
   return c.d;
 }
 
-// Stage coefficients: Solver_rot.f90:223-231 (float() conversions kept).
-__device__ __forceinline__ void stage_coeffs(int istep, int t_order, double dt, double& c0, double& c1,
-                                             double& dt_sub) {
-  if (istep == 1) {
-    c1 = 1.0; c0 = 1.0; dt_sub = dt;
-  } else {
-    c1 = (double)((float)(istep - 1) / (float)(2 * t_order - istep));
-    c0 = 1.0 - c1;
-    dt_sub = c1 * dt;
-  }
-}
-
-// Ghost images of interior index i along one direction of n cells padded by
-// 2 (C indices 2..n+1 interior), as the inverse of the boundary_control
-// source map (Boundary_rot.f90:323-402).  Up to two images per side; image k
-// is packed as (dst + 1) in bits 8k..8k+6 and its mirror flag in bit 8k+7,
-// so the result stays in registers.
-__device__ __forceinline__ unsigned ghost_images(int i, int npad, int tlo, int thi) {
-  unsigned out = 0u;
-  int k = 0;
-  auto put = [&](int dst, bool flip) {
-    out |= ((unsigned)(dst + 1) | (flip ? 0x80u : 0u)) << (8 * k);
-    k++;
-  };
-  if (tlo == 1) {
-    if (i == npad - 4) put(0, false);
-    if (i == npad - 3) put(1, false);
-  } else if (tlo == 2) {
-    if (i == 3) put(0, true);
-    if (i == 2) put(1, true);
-  } else if (tlo == 10) {
-    if (i == 2) { put(0, false); put(1, false); }
-  }
-  if (thi == 1) {
-    if (i == 2) put(npad - 2, false);
-    if (i == 3) put(npad - 1, false);
-  } else if (thi == 2) {
-    if (i == npad - 3) put(npad - 2, true);
-    if (i == npad - 4) put(npad - 1, true);
-  } else if (thi == 10) {
-    if (i == npad - 3) { put(npad - 2, false); put(npad - 1, false); }
-  }
-  return out;
-}
-
 // Store one cell value set (NVT vars) at (ci,cj) and at all its ghost images.
 template <int NDIM, int NVT>
 __device__ __forceinline__ void store_cell(double* U, long long N, int ix, int jx, int ci, int cj,
                                            const double* u, const Bnd& b) {
 #pragma unroll
   for (int n = 0; n < NVT; n++) U[n * N + (long long)cj * ix + ci] = u[n];
-  unsigned gx = ghost_images(ci, ix, b.type[0], b.type[1]);
-  unsigned gy = (NDIM == 2) ? ghost_images(cj, jx, b.type[2], b.type[3]) : 0u;
-  if (gx == 0u && gy == 0u) return;
-  // column images: the own column, then each x image
-  for (int a = -1; a < 4; a++) {
-    int xi;
-    unsigned mx;
-    if (a < 0) {
-      xi = ci;
-      mx = 0u;
-    } else {
-      unsigned e = (gx >> (8 * a)) & 0xffu;
-      if (e == 0u) break;
-      xi = (int)(e & 0x7fu) - 1;
-      mx = (e & 0x80u) ? b.sym[(xi < 2) ? 0 : 1] : 0u;
-    }
-    // row images: the own row (only for x images), then each y image
-    for (int c = -1; c < 4; c++) {
-      int yj;
-      unsigned my;
-      if (c < 0) {
-        if (a < 0) continue;
-        yj = cj;
-        my = 0u;
-      } else {
-        unsigned e = (gy >> (8 * c)) & 0xffu;
-        if (e == 0u) break;
-        yj = (int)(e & 0x7fu) - 1;
-        my = (e & 0x80u) ? b.sym[(yj < 2) ? 2 : 3] : 0u;
-      }
-      unsigned m = mx ^ my;
-      for (int n = 0; n < NVT; n++)
-        U[n * N + (long long)yj * ix + xi] = ((m >> n) & 1u) ? -u[n] : u[n];
-    }
-  }
+  for_each_image(ci, cj, ix, (NDIM == 2) ? jx : 1, b, [&](int xi, int yj, unsigned m) {
+    for (int n = 0; n < NVT; n++) U[n * N + (long long)yj * ix + xi] = ((m >> n) & 1u) ? -u[n] : u[n];
+  });
 }
 
 // One round of the set_IR/cfl chain on one cell (main.f90:32-36 / :44-47):
@@ -536,21 +455,6 @@ __global__ void __launch_bounds__(256) k_round(RoundArgs a) {
 }
 
 // Ghost fill by the composite x-then-y map (Boundary_rot.f90:34-125).
-__device__ __forceinline__ int src_index(int g, int npad, int tlo, int thi, bool& flip) {
-  flip = false;
-  if (g < 2) {
-    if (tlo == 1) return npad - 4 + g;
-    if (tlo == 2) { flip = true; return 3 - g; }
-    return 2;
-  }
-  if (g >= npad - 2) {
-    if (thi == 1) return g - (npad - 4);
-    if (thi == 2) { flip = true; return 2 * npad - 5 - g; }
-    return npad - 3;
-  }
-  return g;
-}
-
 __global__ void __launch_bounds__(256) k_bnd(double* U, int nvt, int ix, int jx, long long N, Bnd b,
                                              int ndim) {
   long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;

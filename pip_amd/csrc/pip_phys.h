@@ -107,6 +107,18 @@ struct Phys {
   int flag_ir_type;  // 0, 1, 2
 };
 
+// Stage coefficients: Solver_rot.f90:223-231 (float() conversions kept).
+PIP_HD void stage_coeffs(int istep, int t_order, double dt, double& c0, double& c1,
+                         double& dt_sub) {
+  if (istep == 1) {
+    c1 = 1.0; c0 = 1.0; dt_sub = dt;
+  } else {
+    c1 = (double)((float)(istep - 1) / (float)(2 * t_order - istep));
+    c0 = 1.0 - c1;
+    dt_sub = c1 * dt;
+  }
+}
+
 // ---------------------------------------------------------------- limiter
 // minmod_func: Util_rot.f90:59-76
 PIP_HD double minmod(double u1, double u2, double u3) {

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/pipgpu.h"
+#include "pip_adjoint.h"
 #include "pip_kernels.h"
 
 using namespace pip;
@@ -54,50 +55,16 @@ struct pip_ctx {
   bool arb_done = false;
   bool top_pending = true;
   int device = 0;
+  // adjoint work buffers (allocated on the first pip_adj_step)
+  double* adjV = nullptr;   // nvt planes
+  double* adjP = nullptr;   // 8 * nvt planes
+  double* lamB = nullptr;   // nvt planes
+  double* lamC = nullptr;   // nvt planes
 };
 
 // --------------------------------------------------------------- helpers
-static int bnd_type(int f) {
-  if ((f >= 3 && f <= 7) || f == 11) return 2;
-  return f;
-}
 
-// set_sym_mhd / set_sym_hd (Boundary_rot.f90:239-320) as sign bit masks.
-static void build_bnd(pip_ctx* c) {
-  const pip_cfg& g = c->cfg;
-  for (int s = 0; s < 4; s++) {
-    c->bnd.type[s] = 0;
-    c->bnd.sym[s] = 0u;
-  }
-  for (int i = 1; i <= 2 * g.ndim; i++) {
-    int dir = (i + 1) / 2, f = g.flag_bnd[i - 1];
-    int sm[9], sh[5];
-    for (int n = 0; n < 9; n++) sm[n] = 1;
-    for (int n = 0; n < 5; n++) sh[n] = 1;
-    if (f == 2) { sm[dir] = -1; sm[5 + dir - 1] = -1; }
-    else if (f == 3) { sm[dir] = -1; sm[6 + dir % 3 - 1] = -1; sm[2 + (dir + 1) % 3 - 1] = -1; }
-    else if (f == 4) { sm[dir] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; sm[2 + dir % 3 - 1] = -1; }
-    else if (f == 5) { sm[dir] = -1; sm[5 + dir - 1] = -1; sm[2 + (dir + 1) % 3 - 1] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; }
-    else if (f == 6) { sm[dir] = -1; sm[6 + dir % 3 - 1] = -1; sm[6 + (dir + 1) % 3 - 1] = -1; }
-    else if (f == 7) { sm[1] = -1; sm[2] = -1; sm[3] = -1; }
-    if (g.flag_divb == 1) {
-      if (f == 2 || f == 5) sm[8] = 1;
-      else if (f == 3 || f == 4 || f == 6 || f == 7) sm[8] = -1;
-    }
-    if (f == 2) sh[dir] = -1;
-    else if (f == 3) { sh[dir] = -1; sh[2 + (dir + 1) % 3 - 1] = -1; }
-    else if (f == 4) { sh[dir] = -1; sh[2 + dir % 3 - 1] = -1; }
-    else if (f == 5) { sh[dir] = -1; sh[2 + (dir + 1) % 3 - 1] = -1; }
-    else if (f == 6) sh[dir] = -1;
-    unsigned m = 0u;
-    for (int n = 0; n < c->nv; n++)
-      if (sm[n] < 0) m |= 1u << n;
-    for (int n = 0; n < 5; n++)
-      if (sh[n] < 0) m |= 1u << (c->nv + n);
-    c->bnd.type[i - 1] = bnd_type(f);
-    c->bnd.sym[i - 1] = m;
-  }
-}
+static void build_bnd(pip_ctx* c) { c->bnd = make_bnd(c->cfg.flag_bnd, c->cfg.ndim, c->nv, c->cfg.flag_divb); }
 
 static DevScalars host_scalars_init() {
   DevScalars s;
@@ -273,6 +240,7 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
   if (cfg->ix < 8) return fail(PIP_ERR_ARG, "ix must be >= 8 (nx >= 4)");
   if (cfg->ndim == 1 && cfg->jx != 1) return fail(PIP_ERR_ARG, "jx must be 1 in 1-D");
   if (cfg->ndim == 2 && cfg->jx < 8) return fail(PIP_ERR_ARG, "jx must be >= 8 (ny >= 4)");
+  if (cfg->ix > kMaxPad || cfg->jx > kMaxPad) return fail(PIP_ERR_ARG, "ix and jx must be <= 32767");
   int want_nv = 8 + ((cfg->ndim == 2 && cfg->flag_divb == 1) ? 1 : 0);
   if (cfg->nvar_m != want_nv) return fail(PIP_ERR_ARG, "nvar_m inconsistent with ndim/flag_divb");
   if (cfg->t_order != 2 && cfg->t_order != 3) return fail(PIP_ERR_UNSUPPORTED, "t_order must be 2 or 3");
@@ -280,7 +248,7 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
     return fail(PIP_ERR_UNSUPPORTED, "flag_IR must be 0 or 3 (set_IR case 3)");
   if (cfg->flag_ir_type < 0 || cfg->flag_ir_type > 2) return fail(PIP_ERR_ARG, "flag_IR_type in 0..2");
   for (int s = 0; s < 2 * cfg->ndim; s++) {
-    int t = bnd_type(cfg->flag_bnd[s]);
+    int t = bnd_kind(cfg->flag_bnd[s]);
     if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
   }
   if (!(cfg->dx > 0) || (cfg->ndim == 2 && !(cfg->dy > 0))) return fail(PIP_ERR_ARG, "dx, dy must be > 0");
@@ -349,6 +317,10 @@ int pip_destroy(pip_ctx* c) {
   (void)hipFree(c->arb_heat);
   (void)hipFree(c->ion_pot);
   (void)hipFree(c->sc);
+  (void)hipFree(c->adjV);
+  (void)hipFree(c->adjP);
+  (void)hipFree(c->lamB);
+  (void)hipFree(c->lamC);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PIP_OK;
@@ -617,6 +589,155 @@ int pip_time_steps(pip_ctx* c, int nsteps, double* ms_total, double* ms_stage1, 
 
 }  // extern "C"
 
+// ---------------------------------------------------------------- adjoint
+static dim3 cell_grid(const pip_ctx* c) { return dim3((unsigned)((c->N + 255) / 256)); }
+
+static int adj_check(const pip_ctx* c) {
+  if (c->cfg.ndim != 2 || c->nv != ADJ_NV) return fail(PIP_ERR_UNSUPPORTED, "adjoint needs ndim = 2 and flag_divb = 1");
+  if (c->cfg.flag_ir != 0) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers flag_IR = 0");
+  if (c->cfg.t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers SSP-RK2 (t_order = 2)");
+  for (int s = 0; s < 4; s++)
+    if (c->bnd.type[s] == 10) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers periodic and mirror boundaries");
+  return PIP_OK;
+}
+
+static int adj_alloc(pip_ctx* c) {
+  if (c->adjV) return PIP_OK;
+  const size_t pb = sizeof(double) * (size_t)c->N * c->nvt;
+  HIPCHK(hipMalloc(&c->adjV, pb));
+  HIPCHK(hipMalloc(&c->adjP, 8 * pb));
+  HIPCHK(hipMalloc(&c->lamB, pb));
+  HIPCHK(hipMalloc(&c->lamC, pb));
+  return PIP_OK;
+}
+
+static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* lam_out, double* lam_in,
+                        double* lam_U0, double* lam_ac, const double* dtc) {
+  AdjArgs a;
+  a.Uin = Uin;
+  a.lam_out = lam_out;
+  a.lam_in = lam_in;
+  a.lam_U0 = lam_U0;
+  a.lam_ac = lam_ac;
+  a.ac = c->cfg.ac_field ? c->ac : nullptr;
+  a.V = c->adjV;
+  a.P = c->adjP;
+  a.dtc = dtc;
+  a.ix = c->cfg.ix;
+  a.jx = c->cfg.jx;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.N = c->N;
+  a.istep = istep;
+  a.t_order = c->cfg.t_order;
+  a.dx = c->cfg.dx;
+  a.dy = c->cfg.dy;
+  a.col = c->cfg.col;
+  a.safety = c->cfg.safety;
+  a.db_clean = c->cfg.db_clean;
+  a.ph = c->ph;
+  a.bnd = c->bnd;
+  return a;
+}
+
+static int adj_stage(pip_ctx* c, const AdjArgs& a) {
+  hipLaunchKernelGGL(k_adj_cell, cell_grid(c), dim3(256), 0, c->stream, a);
+  const long long fx = (long long)(c->nx + 1) * c->ny, fy = (long long)c->nx * (c->ny + 1);
+  const dim3 gx((unsigned)((fx + 255) / 256)), gy((unsigned)((fy + 255) / 256));
+  hipLaunchKernelGGL(k_adj_faces<0>, gx, dim3(256), 0, c->stream, a, 0);
+  hipLaunchKernelGGL(k_adj_faces<1>, gx, dim3(256), 0, c->stream, a, 0);
+  hipLaunchKernelGGL(k_adj_faces<0>, gy, dim3(256), 0, c->stream, a, 1);
+  hipLaunchKernelGGL(k_adj_faces<1>, gy, dim3(256), 0, c->stream, a, 1);
+  hipLaunchKernelGGL(k_adj_gather, cell_grid(c), dim3(256), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+static int fill(pip_ctx* c, double* x, long long n) {
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, x, n, 0.0);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+extern "C" {
+
+int pip_set_state_device(pip_ctx* c, const double* U) {
+  if (!c || !U) return fail(PIP_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(c->U, U, sizeof(double) * (size_t)c->N * c->nvt, hipMemcpyDeviceToDevice, c->stream));
+  c->top_pending = true;
+  return PIP_OK;
+}
+
+int pip_get_state_device(pip_ctx* c, double* U) {
+  if (!c || !U) return fail(PIP_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(U, c->U, sizeof(double) * (size_t)c->N * c->nvt, hipMemcpyDeviceToDevice, c->stream));
+  return PIP_OK;
+}
+
+int pip_set_ac_device(pip_ctx* c, const double* ac) {
+  if (!c || !ac) return fail(PIP_ERR_ARG, "null argument");
+  if (!c->cfg.ac_field) return fail(PIP_ERR_STATE, "context was created with ac_field = 0");
+  HIPCHK(hipMemcpyAsync(c->ac, ac, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToDevice, c->stream));
+  return PIP_OK;
+}
+
+int pip_record_steps(pip_ctx* c, int n, double* hist) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (n < 0 || (n > 0 && !hist)) return fail(PIP_ERR_ARG, "n < 0 or null history");
+  if (c->top_pending) TRY(loop_top(c, 1.0e300, (long long)1 << 62));
+  for (int k = 0; k < n; k++) {
+    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
+    TRY(run_stages(c, 2, nullptr));
+    TRY(launch_finalize(c, 1, 0));
+  }
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+int pip_replay_steps(pip_ctx* c, int n, const double* hist, double* Un, double* U1) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (n < 0 || (n > 0 && !hist)) return fail(PIP_ERR_ARG, "n < 0 or null history");
+  if (c->cfg.t_order != 2 && U1) return fail(PIP_ERR_UNSUPPORTED, "stage saves cover t_order = 2");
+  const size_t sb = sizeof(double) * (size_t)c->N * c->nvt;
+  TRY(set_loop(c, 1.0e300, (long long)1 << 62, 1, -1));
+  c->top_pending = false;
+  for (int k = 0; k < n; k++) {
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
+    if (Un) HIPCHK(hipMemcpyAsync(Un + (size_t)k * c->N * c->nvt, c->U, sb, hipMemcpyDeviceToDevice, c->stream));
+    TRY(run_stages(c, 2, nullptr));
+    if (U1) HIPCHK(hipMemcpyAsync(U1 + (size_t)k * c->N * c->nvt, c->U1, sb, hipMemcpyDeviceToDevice, c->stream));
+    TRY(launch_finalize(c, 1, 0));
+  }
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+int pip_adj_seed(pip_ctx* c, double* lam) {
+  if (!c || !lam) return fail(PIP_ERR_ARG, "null argument");
+  hipLaunchKernelGGL(k_adj_fold_seed, cell_grid(c), dim3(256), 0, c->stream, lam, c->nvt, c->cfg.ix, c->cfg.jx,
+                     c->N, c->bnd);
+  hipLaunchKernelGGL(k_zero_ghosts, cell_grid(c), dim3(256), 0, c->stream, lam, c->nvt, c->cfg.ix, c->cfg.jx, c->N);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* dtc, double* lam, double* lam_ac) {
+  if (!c || !Un || !U1 || !dtc || !lam) return fail(PIP_ERR_ARG, "null argument");
+  TRY(adj_check(c));
+  TRY(adj_alloc(c));
+  const long long nn = c->N * c->nvt;
+  TRY(fill(c, c->lamB, nn));
+  TRY(fill(c, c->lamC, nn));
+  // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1)
+  TRY(adj_stage(c, adj_args(c, 2, U1, lam, c->lamB, c->lamC, lam_ac, dtc)));
+  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n)
+  TRY(adj_stage(c, adj_args(c, 1, Un, c->lamB, c->lamC, c->lamC, lam_ac, dtc)));
+  HIPCHK(hipMemcpyAsync(lam, c->lamC, sizeof(double) * (size_t)nn, hipMemcpyDeviceToDevice, c->stream));
+  return PIP_OK;
+}
+
+}  // extern "C"
+
 // ---------------------------------------------------------------- ensemble
 struct pip_ens {
   pip_cfg base;
@@ -691,11 +812,13 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   if (base->flag_ir != 0 || base->ac_field) return fail(PIP_ERR_UNSUPPORTED, "ensemble: flag_IR = 0, constant alpha_c");
   if (base->t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "ensemble: t_order = 2");
   for (int s = 0; s < 2; s++) {
-    int t = bnd_type(base->flag_bnd[s]);
+    int t = bnd_kind(base->flag_bnd[s]);
     if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
   }
-  for (int m = 0; m < n; m++)
+  for (int m = 0; m < n; m++) {
     if (members[m].nx < 4 || !(members[m].dx > 0)) return fail(PIP_ERR_ARG, "member nx >= 4 and dx > 0 required");
+    if (members[m].nx + 4 > kMaxPad) return fail(PIP_ERR_ARG, "member nx must be <= 32763");
+  }
   pip_ens* e = new pip_ens();
   e->base = *base;
   e->n = n;

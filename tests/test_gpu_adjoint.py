@@ -1,0 +1,170 @@
+"""GPU checks of the discrete adjoint (pip_amd.adjoint over pip_adj_step).
+
+The gradient of J = 1/2 sum_interior |U_T - U*|^2 with respect to the
+per-cell collision coefficient field and the initial velocity is checked
+against central finite differences of the frozen-dt replay (the map the
+adjoint differentiates), on the 2-D two-fluid Alfven-wave damping problem.
+
+Finite-difference steps are relative (eps = 1e-6): ac_k -> ac_k (1 +- eps)
+and v_z,k -> v_z,k +- eps * A with A the wave amplitude (v_z is the wave's
+velocity component; v_x = v_y = 0 exactly at the base point, where minmod
+has no derivative).  An absolute 1e-6 velocity step would be 1e-3 of the
+wave and larger than the second differences the minmod limiter compares
+(~1e-7 at 512 cells per wavelength), so the difference quotient would
+straddle limiter switches instead of measuring the derivative.  J(+) - J(-) is summed as
+sum (U+ - U-) . ((U+ + U-)/2 - U*), which is exact in the cells the
+perturbation does not reach.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from pip_amd import problems  # noqa: E402
+from pip_amd.solver import PIPSolver  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+AMP = 1.0e-3
+
+
+def _setup(n, nsteps, ckpt, seed=11):
+    from pip_amd.adjoint import TwoFluidRun
+    cfg, uh, um, ac = problems.alfven_damping(n, amp=AMP, seed=seed)
+    run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt)
+    dev = torch.device("cuda:0")
+    # primitive fields of the initial state (both fluids share v)
+    f_n, f_p, f_pn, f_pp = problems.split_fractions(0.5)
+    X, Y = problems._grid(cfg)
+    base = dict(rho_n=torch.tensor(uh[0], device=dev), rho_p=torch.tensor(um[0], device=dev),
+                p_n=torch.full((cfg.jx, cfg.ix), f_pn * 0.1, dtype=torch.float64, device=dev),
+                p_p=torch.full((cfg.jx, cfg.ix), f_pp * 0.1, dtype=torch.float64, device=dev),
+                B=torch.tensor(um[5:8], device=dev), psi=torch.tensor(um[8], device=dev))
+    v0 = torch.tensor(np.stack([um[1] / um[0], um[2] / um[0], um[3] / um[0]]), device=dev)
+    return cfg, run, base, v0, torch.tensor(ac, device=dev), (uh, um, ac)
+
+
+def build_U0(cfg, base, v):
+    """pv2cq_mhd / pv2cq_hd (Scheme_rot.f90:30-77) from the shared velocity."""
+    g = cfg.gm
+    rp, rn = base["rho_p"], base["rho_n"]
+    v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]
+    B = base["B"]
+    b2 = (B[0] * B[0] + B[1] * B[1]) + B[2] * B[2]
+    Um = [rp, rp * v[0], rp * v[1], rp * v[2], (base["p_p"] / (g - 1.0) + (0.5 * rp) * v2) + 0.5 * b2,
+          B[0], B[1], B[2], base["psi"]]
+    Uh = [rn, rn * v[0], rn * v[1], rn * v[2], base["p_n"] / (g - 1.0) + (0.5 * rn) * v2]
+    return torch.stack(Um + Uh).contiguous()
+
+
+def interior_mask(cfg, dev):
+    m = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
+    m[2:-2, 2:-2] = 1.0
+    return m
+
+
+def target_state(cfg, arrays, nsteps, seed=12):
+    """Final snapshot of a run with a different collision field and a 10 %
+    larger initial wave (plain fused loop)."""
+    uh, um, ac = arrays
+    g2 = problems.smooth_field(cfg, seed)
+    ac_t = ac * np.exp(0.5 * g2)
+    uh_t, um_t = problems.alfven_damping(cfg.nx, amp=1.1 * AMP)[1:3]
+    s = PIPSolver(cfg, uh_t, um_t, ac_t)
+    s.advance(nsteps)
+    uh_T, um_T = s.state()
+    s.close()
+    return torch.tensor(np.concatenate([um_T, uh_T]), device="cuda:0")
+
+
+def test_replay_is_bitwise_the_recorded_run():
+    print("case replay vs record n=64 steps=45", flush=True)
+    cfg, run, base, v0, ac, _ = _setup(64, 45, 7)
+    U0 = build_U0(cfg, base, v0)
+    out, ckpts = run.forward_record(U0, ac)
+    assert len(ckpts) == 7
+    rep = run.replay(U0, ac)
+    assert torch.equal(out, rep)
+    dts = run.dt_history()
+    assert np.all(dts > 0) and len(dts) == 45
+    run.close()
+
+
+def test_record_matches_fused_loop():
+    """forward_record runs the time loop of pip_advance (same kernels, same
+    dt sequence); only the final rewrite round differs (ulp level)."""
+    print("case record vs pip_advance n=64 steps=30", flush=True)
+    cfg, run, base, v0, ac, (uh, um, acn) = _setup(64, 30, 10)
+    out, _ = run.forward_record(build_U0(cfg, base, v0), ac)
+    s = PIPSolver(cfg, uh, um, acn)
+    s.advance(30)
+    uh_T, um_T = s.state()
+    ref = torch.tensor(np.concatenate([um_T, uh_T]), device="cuda:0")
+    m = interior_mask(cfg, ref.device)
+    err = ((out - ref).abs() * m).sum() / ((ref.abs() * m).sum())
+    assert err < 1e-13, float(err)
+    run.close()
+
+
+def _fd_check(n, nsteps, ckpt, ncells, tol, seed=5):
+    cfg, run, base, v0, ac, arrays = _setup(n, nsteps, ckpt)
+    dev = ac.device
+    Ustar = target_state(cfg, arrays, nsteps)
+    mask = interior_mask(cfg, dev)
+    v = v0.clone().requires_grad_(True)
+    acv = ac.clone().requires_grad_(True)
+    UT = run.run(build_U0(cfg, base, v), acv)
+    J = 0.5 * (((UT - Ustar) ** 2) * mask).sum()
+    J.backward()
+    g_ac, g_v = acv.grad.clone(), v.grad.clone()
+    # gradient support: interior only
+    assert float(g_ac[:2].abs().max()) == 0.0 and float(g_ac[:, :2].abs().max()) == 0.0
+
+    def dJ(Up, Um):
+        return float((((Up - Um) * (0.5 * (Up + Um) - Ustar)) * mask).sum())
+
+    rng = np.random.default_rng(seed)
+    cells = [(int(rng.integers(2, cfg.jx - 2)), int(rng.integers(2, cfg.ix - 2))) for _ in range(ncells)]
+    eps = 1.0e-6
+    errs_ac, errs_v = [], []
+    with torch.no_grad():
+        U0 = build_U0(cfg, base, v0)
+        for c, (j, i) in enumerate(cells):
+            if c % 16 == 0:
+                print(f"  FD cell {c}/{len(cells)}", flush=True)
+            h = eps * float(ac[j, i])
+            acp, acm = ac.clone(), ac.clone()
+            acp[j, i] += h
+            acm[j, i] -= h
+            fd = dJ(run.replay(U0, acp), run.replay(U0, acm)) / (2 * h)
+            ad = float(g_ac[j, i])
+            errs_ac.append(abs(fd - ad) / max(abs(fd), 1e-300))
+            # the wave component; v_x = v_y = 0 exactly at the base point,
+            # which puts their limiters at minmod's kink (no derivative)
+            comp = 2
+            h = eps * AMP
+            vp, vm = v0.clone(), v0.clone()
+            vp[comp, j, i] += h
+            vm[comp, j, i] -= h
+            fd = dJ(run.replay(build_U0(cfg, base, vp), ac), run.replay(build_U0(cfg, base, vm), ac)) / (2 * h)
+            ad = float(g_v[comp, j, i])
+            errs_v.append(abs(fd - ad) / max(abs(fd), 1e-300))
+    print(f"FD check n={n} steps={nsteps}: ac max rel err {max(errs_ac):.3e}, "
+          f"v max rel err {max(errs_v):.3e}", flush=True)
+    run.close()
+    assert max(errs_ac) <= tol, errs_ac
+    assert max(errs_v) <= tol, errs_v
+
+
+def test_adjoint_fd_small():
+    print("case adjoint FD n=48 steps=36 ckpt=7", flush=True)
+    _fd_check(48, 36, 7, 16, 1e-6)
+
+
+def test_adjoint_fd_alfven_512():
+    """512x512 periodic, 200 SSP-RK2 steps, checkpoints every 20 steps,
+    64 random cells."""
+    print("case adjoint FD n=512 steps=200 ckpt=20", flush=True)
+    _fd_check(512, 200, 20, 64, 1e-6)

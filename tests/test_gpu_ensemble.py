@@ -21,16 +21,21 @@ def _members(n, nx_cap=None):
     return out
 
 
+# every run is capped at MAXS steps (the members need about 3000 to reach
+# t = 0.1), so a wrong dt cannot keep a launch running for long
+MAXS = 20000
+
+
 @pytest.mark.parametrize("graph,tile", [(True, 256), (False, 64), (True, 512)])
 def test_ensemble_matches_plain_runs(graph, tile):
     print("case ensemble 10 members graph", graph, "tile", tile, flush=True)
     mem = _members(10)
-    ens = Ensemble(mem, tend=0.1, tile=tile)
+    ens = Ensemble(mem, tend=0.1, max_steps=MAXS, tile=tile)
     ms = ens.run(graph=graph)
     print("ensemble ms", ms, flush=True)
     for k, (cfg, uh, um) in enumerate(mem):
         s = PIPSolver(cfg, uh, um)
-        n, t = s.advance(tend=0.1)
+        n, t = s.advance(nsteps=MAXS, tend=0.1)
         r = ens.result(k)
         assert r["status"] == 0
         assert r["nt"] == n and r["time"] == t, (k, r, n, t)
@@ -44,7 +49,7 @@ def test_ensemble_matches_plain_runs(graph, tile):
 def test_ensemble_rerun_after_reset_is_identical():
     print("case ensemble reset/rerun", flush=True)
     mem = _members(6, nx_cap=600)
-    ens = Ensemble(mem, tend=0.05)
+    ens = Ensemble(mem, tend=0.05, max_steps=MAXS)
     ens.run(graph=True)
     a = [ens.state(k) for k in range(6)]
     ens.reset()
