@@ -11,6 +11,12 @@ roofline = algorithmic HBM bytes of the dominant kernel (the stage-2 kernel:
 3*nv*8 B per cell) / its average duration measured with hipEvents on the
 library's stream over the same timed steps; cpu_baseline = the CPU oracle
 (single core) on a bounded sample of the same workload.
+Two more objects compare the training-system patterns with plain runs:
+batched = 256 uneven 1-D shock tubes in one persistent hipGraph launch
+against the same 256 runs as sequential plain pip_advance calls (results
+compared bit for bit); adjoint = gradient (checkpointed forward + reverse
+sweep) of the 512^2 Alfven-wave damping run (200 steps, checkpoints every
+20) against the plain forward run.
 """
 from __future__ import annotations
 
@@ -20,6 +26,8 @@ import os
 import re
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -50,6 +58,92 @@ def cpu_baseline(n: int, steps: int):
                       f"{steps} steps, {dt:.1f} s on one core"}
 
 
+def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000):
+    """Ensemble of n_members 1-D Brio-Wu-type shock tubes (nx log-uniform in
+    [256, 4096], xi_n, alpha_c and the pressure ratio drawn from seed 7) run
+    to t = tend: one persistent launch vs sequential plain launches."""
+    import torch
+    from pip_amd import autotune, problems
+    from pip_amd.ensemble import Ensemble
+    from pip_amd.solver import PIPSolver
+    params = problems.ensemble_members(n_members, seed=7)
+    mem = [problems.ensemble_member(p)[:3] for p in params]
+    ens = Ensemble(mem, tend=tend, max_steps=max_steps)
+    tile, tune_ms = autotune.tune_ensemble(ens, f"ens:{n_members}:seed7:tend{tend}")
+    ens.reset()
+    ens.run(graph=True)  # builds and caches the graph
+    ens.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens_ms = ens.run(graph=True)
+    wall_ens = time.perf_counter() - t0
+    updates = 0
+    wall_seq = 0.0
+    identical = True
+    for k, (cfg, uh, um) in enumerate(mem):
+        s = PIPSolver(cfg, uh, um)
+        s.sync()
+        t0 = time.perf_counter()
+        n, t = s.advance(nsteps=max_steps, tend=tend)
+        wall_seq += time.perf_counter() - t0
+        r = ens.result(k)
+        updates += cfg.nx * n
+        eh, em = ens.state(k)
+        ph, pm = s.state()
+        identical &= (r["nt"] == n and r["time"] == t and np.array_equal(em[:, :, 2:-2], pm[:, :, 2:-2])
+                      and np.array_equal(eh[:, :, 2:-2], ph[:, :, 2:-2]))
+        s.close()
+    ens.close()
+    return {"workload": f"{n_members} x 1-D two-fluid shock tube, nx log-uniform [256,4096], t_end={tend}",
+            "cell_updates": updates, "tile": tile, "tile_timings_ms": tune_ms,
+            "ensemble_s": wall_ens, "ensemble_device_ms": ens_ms, "sequential_plain_s": wall_seq,
+            "ensemble_value": updates / wall_ens, "sequential_value": updates / wall_seq,
+            "unit": "cell-updates/s", "speedup": wall_seq / wall_ens, "bitwise_identical": bool(identical)}
+
+
+def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
+    """Gradient of J = |U_T|^2/2 (interior) w.r.t. the initial state and the
+    per-cell alpha_c field vs the plain forward run (pip_advance)."""
+    import torch
+    from pip_amd import problems
+    from pip_amd.adjoint import TwoFluidRun
+    from pip_amd.solver import PIPSolver
+    cfg, uh, um, ac = problems.alfven_damping(n)
+    s = PIPSolver(cfg, uh, um, ac)
+    s.advance(5)  # warm-up
+    s.close()
+    s = PIPSolver(cfg, uh, um, ac)
+    s.sync()
+    t0 = time.perf_counter()
+    s.advance(nsteps)
+    t_fwd = time.perf_counter() - t0
+    s.close()
+    dev = torch.device("cuda:0")
+    run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt)
+    U0 = torch.tensor(np.concatenate([um, uh]), device=dev)
+    acT = torch.tensor(ac, device=dev)
+    mask = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
+    mask[2:-2, 2:-2] = 1.0
+    times = []
+    for _ in range(2):  # the first pass allocates the adjoint buffers
+        U = U0.clone().requires_grad_(True)
+        a = acT.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        UT = run.run(U, a)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        UT.backward(UT.detach() * mask)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
+    t_rec, t_bwd = times[-1]
+    run.close()
+    return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
+            "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
+            "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +153,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=str, default="auto", help="TXxTY, or auto (autotuner)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
+    ap.add_argument("--only", choices=["batched", "adjoint"], help="run one extra leg alone (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -68,6 +164,10 @@ def main():
     import torch  # device plumbing only: sync around the timed region
     from pip_amd import autotune, problems
     from pip_amd.solver import PIPSolver
+
+    if args.only:
+        print(json.dumps({args.only: batched_leg() if args.only == "batched" else adjoint_leg()}))
+        return
 
     cfg, uh, um, ac = problems.orszag_tang(args.n)
     if args.tile == "auto":
@@ -118,6 +218,9 @@ def main():
                      "stage1_achieved_GBs": bytes_stage1 / t1 / 1e9},
         "hbm_roofline_cell_updates_frac": value * (5 * nv * 8) / (HBM_PEAK_GBS * 1e9),
     }
+    if not args.no_extra:
+        out["batched"] = batched_leg()
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps)
     print(json.dumps(out))

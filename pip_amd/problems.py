@@ -175,19 +175,23 @@ def periodic_fill(a: np.ndarray) -> np.ndarray:
 
 
 def alfven_damping(n: int = 512, xi_n: float = 0.5, amp: float = 1.0e-3,
-                   ac_mean: float = 50.0, ac_sigma: float = 0.5, seed: int = 11) -> Arrays:
+                   ac_mean: float = 50.0, ac_sigma: float = 0.5, seed: int = 11,
+                   phase: float = 0.3) -> Arrays:
     """2-D linear Alfven wave on periodic [0,1]^2, rho=1, p=0.1, B0=(1,0.5),
     polarised along z, with an oblique wave vector k = 2 pi (1, 1) so that
     no state variable of the wave is uniform along a grid direction (an
     exactly uniform direction puts every minmod limiter of that sweep at its
-    non-differentiable point X = Y = 0); per-cell log-normal alpha_c field
+    non-differentiable point X = Y = 0), and a phase offset so that the zero
+    crossings do not fall midway between cell centres (there |X| = |Y|
+    exactly, a tie between the two one-sided slopes); per-cell log-normal
+    alpha_c field
     ac = exp(ln(mean) - sigma^2/2 + sigma*g)."""
     cfg = PIPConfig(ndim=2, nx=n, ny=n, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0,
                     col=ac_mean, flag_divb=1, ac_field=True, flag_bnd=(1, 1, 1, 1))
     X, Y = _grid(cfg)
     z = np.zeros_like(X)
     f_n, f_p, _, _ = split_fractions(xi_n)
-    s = np.sin(2.0 * math.pi * (X + Y))
+    s = np.sin(2.0 * math.pi * (X + Y) + phase)
     dvz = amp * s
     dbz = -amp * math.sqrt(f_p * 1.0) * s
     U_h, U_m = _two_fluid(cfg, xi_n, z + 1.0, z + 0.1, z, z, dvz, z + 1.0, z + 0.5, dbz)
