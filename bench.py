@@ -45,6 +45,55 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def _pmc_rows(d: str, counter: str):
+    """(kernel name, value) per dispatch of one counter from the rocprofv3
+    --pmc CSVs under d, in dispatch order."""
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") == counter:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def write_pmc_summary(d: str, cmds: dict):
+    """HBM bytes per stage-2 k_stage launch from four rocprofv3 --pmc runs
+    (d/fetch, d/write: the main workload; d/copy_fetch, d/copy_write: the
+    1 GiB copy).  FETCH_SIZE / WRITE_SIZE are in KiB per dispatch; each is
+    scaled by known bytes / counted bytes of the copy kernel."""
+    def stage2(counter, sub):
+        v = [val for k, val in _pmc_rows(os.path.join(d, sub), counter) if "k_stage" in k]
+        return v[1::2], v[0::2]  # each step launches stage 1 then stage 2
+
+    def copy_kib(counter, sub):
+        v = [val for k, val in _pmc_rows(os.path.join(d, sub), counter) if "k_stage" not in k]
+        return max(v) if v else float("nan")
+    gib_kib = float(1 << 20)
+    f_scale = gib_kib / copy_kib("FETCH_SIZE", "copy_fetch")
+    w_scale = gib_kib / copy_kib("WRITE_SIZE", "copy_write")
+    f2, f1 = stage2("FETCH_SIZE", "fetch")
+    w2, w1 = stage2("WRITE_SIZE", "write")
+    mean = lambda x: sum(x) / len(x)
+    fetch_b = mean(f2) * 1024 * f_scale
+    write_b = mean(w2) * 1024 * w_scale
+    ncell, nv = 1024 * 1024, 14
+    lines = ["# k_stage (stage 2), Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
+             "Commands (each its own pass):", ""]
+    lines += [f"    {c}" for c in cmds.values()]
+    lines += ["", f"copy calibration: FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
+              "(1 GiB read + 1 GiB written per copy)", "",
+              f"stage-2 launches: {len(f2)}; mean FETCH_SIZE {mean(f2):.1f} KiB, WRITE_SIZE {mean(w2):.1f} KiB",
+              f"stage-1 launches: {len(f1)}; mean FETCH_SIZE {mean(f1):.1f} KiB, WRITE_SIZE {mean(w1):.1f} KiB", "",
+              f"stage-2 HBM bytes per launch (scaled): read {fetch_b:.4e}, write {write_b:.4e}",
+              f"algorithmic bytes per launch: read {2 * nv * 8 * ncell:.4e}, write {nv * 8 * ncell:.4e}", "",
+              f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}", ""]
+    open(PROFILE_PMC, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
 def cpu_baseline(n: int, steps: int):
     from oracle.oracle import Oracle
     from pip_amd import problems
@@ -154,8 +203,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=str, default="auto", help="TXxTY, or auto (autotuner)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
-    ap.add_argument("--only", choices=["batched", "adjoint"], help="run one extra leg alone (profiling)")
+    ap.add_argument("--pmc-summary", metavar="DIR", help="write the HBM-traffic profile summary from "
+                    "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write) and exit")
+    ap.add_argument("--only", choices=["batched", "adjoint", "copy"],
+                    help="run one extra leg alone (profiling); copy = 1 GiB device copy that "
+                         "calibrates the HBM byte counters")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pm = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
+              "-d {d}/{sub} -o run -- python3 /root/repo/bench.py {a}")
+        main_args = "--steps 20 --warmup 5 --no-cpu --no-extra --tile 32x7"
+        cmds = {sub: pm.format(c=c, d=args.pmc_summary, sub=sub, a=a) for sub, c, a in (
+            ("fetch", "FETCH_SIZE", main_args), ("write", "WRITE_SIZE", main_args),
+            ("copy_fetch", "FETCH_SIZE", "--only copy"), ("copy_write", "WRITE_SIZE", "--only copy"))}
+        write_pmc_summary(args.pmc_summary, cmds)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
               "multi-GPU sharding is a design, not a measured path", file=sys.stderr)
@@ -165,6 +227,15 @@ def main():
     from pip_amd import autotune, problems
     from pip_amd.solver import PIPSolver
 
+    if args.only == "copy":
+        n = 1 << 27  # 2^27 doubles = 1 GiB per buffer
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n}}))
+        return
     if args.only:
         print(json.dumps({args.only: batched_leg() if args.only == "batched" else adjoint_leg()}))
         return

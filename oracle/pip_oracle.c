@@ -26,6 +26,7 @@
 #define MU_P 0.5     /* PIP_rot.f90:36 */
 #define MU_N 1.0     /* PIP_rot.f90:37 */
 #define SGL(x) ((double)(float)(x))
+#define PI 3.1415926535897932 /* Parameters.f90:2 */
 
 struct orc_state {
     orc_cfg c;
@@ -37,6 +38,7 @@ struct orc_state {
     long nt;
     int sym_m[4][9], sym_h[4][5];
     double *stage_save;    /* optional copy of the stage-1 result (test hook) */
+    double dt_coll_i;      /* set_dt_coll value of the current stage        */
     /* scratch */
     double *V, *VL, *VR, *Fm, *Fh, *Sm, *Sh, *Um0, *Uh0, *tmp;
 };
@@ -740,11 +742,46 @@ void orc_set_ir(orc_state *s) {
     }
 }
 
-/* set_coefficients: Solver_rot.f90:95-121 (collision: flag_col=1 constant) */
+/* set_collisional: PIP_rot.f90:44-97, col_type = flag_col-1 (:21-25).
+ * Cases 1..6 take T_n, T_p from get_Te_* (which rewrite E on every cell)
+ * and the drift v_n - v_p from get_vel_diff (Scheme_rot.f90:988-999). */
+static void set_collisional(orc_state *s) {
+    const orc_cfg *c = &s->c;
+    int N = s->N;
+    int ct = (c->flag_col >= 2 && c->flag_col <= 7) ? c->flag_col - 1 : 0;
+    if (ct == 0) {
+        if (!c->ac_field)
+            for (int k = 0; k < N; k++) s->ac[k] = c->col; /* case 0 */
+        return;
+    }
+    double *Te_m = s->tmp, *Te_h = s->tmp + N;
+    get_pr_mhd(s, s->Um, NULL, Te_m); /* get_Te_MHD :57 */
+    get_pr_hd(s, s->Uh, NULL, Te_h);  /* get_Te_HD  :58 */
+    for (int k = 0; k < N; k++) {
+        double tt = Te_h[k] + Te_m[k];
+        double a = c->col * sqrt(tt / 2.0);
+        if (ct == 1) a = a / sqrt(c->beta / 2.0 * c->gm);
+        else if (ct >= 3) {
+            double vd[3];
+            for (int i = 0; i < 3; i++)
+                vd[i] = s->Uh[(i + 1) * N + k] / s->Uh[k] - s->Um[(i + 1) * N + k] / s->Um[k];
+            double v2 = (vd[0] * vd[0] + vd[1] * vd[1]) + vd[2] * vd[2];
+            a = a * sqrt(1.0 + 9.0 * PI / 64.0 * c->gm / 2.0 * v2 / tt);
+            if (ct == 3) a = a / sqrt(c->beta / 2.0 * c->gm);
+            else if (ct == 5)
+                a = a / sqrt(c->beta / 2.0 * c->gm) *
+                    orc_pow(c->beta / 2.0 * c->gm / tt / 2.0 + c->gm * PI / 16.0 * v2, 0.125);
+            else if (ct == 6)
+                a = a / orc_pow(tt / 2.0 + c->gm * PI / 16.0 * v2, 0.125);
+        }
+        s->ac[k] = a;
+    }
+}
+
+/* set_coefficients: Solver_rot.f90:95-121 */
 void orc_set_coefficients(orc_state *s, int flag) {
     const orc_cfg *c = &s->c;
-    if (flag == 0 && c->flag_col == 1 && !c->ac_field)
-        for (int k = 0; k < s->N; k++) s->ac[k] = c->col; /* set_collisional case 0 */
+    if (c->flag_col >= 2 || (flag == 0 && c->flag_col == 1)) set_collisional(s);
     if (c->flag_ir >= 2 || (flag == 0 && c->flag_ir == 1)) orc_set_ir(s);
 }
 
@@ -791,7 +828,8 @@ double orc_cfl(orc_state *s, int into_global_dt) {
         mm = dmax(mm, s->Um[k] * s->ac[k]);
         mh = dmax(mh, s->Uh[k] * s->ac[k]);
     }
-    dttemp = dmin(dttemp, dmin(c->safety, 0.3) / dmax(mm, mh));
+    if (c->flag_pip_imp == 1) dttemp = dmin(dttemp, c->pip_imp_factor / dmax(mm, mh));
+    else dttemp = dmin(dttemp, dmin(c->safety, 0.3) / dmax(mm, mh));
     /* cfl_pip_ir :315-327 */
     if (c->flag_ir >= 1) {
         double mr = -HUGE_VAL, mi = -HUGE_VAL;
@@ -840,14 +878,40 @@ void orc_source_pip(orc_state *s, double *S_h, double *S_m) {
         double um1 = s->Um[k], uh1 = s->Uh[k];
         double dS[5];
         dS[0] = 0.0;
-        dS[1] = ac * (um1 * s->Uh[N + k] - uh1 * s->Um[N + k]);
-        dS[2] = ac * (um1 * s->Uh[2 * N + k] - uh1 * s->Um[2 * N + k]);
-        dS[3] = ac * (um1 * s->Uh[3 * N + k] - uh1 * s->Um[3 * N + k]);
-        dS[4] = ac * nde * de * (0.5 * ((nvx * nvx - vx * vx) + (nvy * nvy - vy * vy) + (nvz * nvz - vz * vz))
-                                 + 3.0 / c->gm / 2.0 * (nte - te));
-        for (int n = 0; n < 5; n++) {
-            S_h[n * N + k] = S_h[n * N + k] - dS[n];
-            S_m[n * N + k] = S_m[n * N + k] + dS[n];
+        if (c->flag_pip_imp == 1) {
+            /* implicit exchange (:1333-1358): integrated over dt_coll_i and
+             * applied to the stage base U0, not to S */
+            double dtc = s->dt_coll_i;
+            double temp = dtc * (uh1 + um1);
+            double ex = orc_exp(-ac * temp);
+            dS[1] = (um1 * s->Uh[N + k] - uh1 * s->Um[N + k]) * (1.0 - ex) / temp;
+            dS[2] = (um1 * s->Uh[2 * N + k] - uh1 * s->Um[2 * N + k]) * (1.0 - ex) / temp;
+            dS[3] = (um1 * s->Uh[3 * N + k] - uh1 * s->Um[3 * N + k]) * (1.0 - ex) / temp;
+            double lambda = ac * (nde + de);
+            double kapper = 3.0 * (c->gm - 1.0) * ac * (MU_P * nde + MU_N * de);
+            double B = -1.0 / 3.0 * c->gm * kapper / ((nde + de) * (kapper - lambda)) *
+                       (de * ((nvx - vx) * vx + (nvy - vy) * vy + (nvz - vz) * nvz) +
+                        nde * ((nvx - vx) * nvx + (nvy - vy) * nvy + (nvz - vz) * nvz));
+            double D = -c->gm / 6.0 * kapper * (de - nde) / ((de + nde) * (kapper - 2.0 * lambda)) *
+                       (((nvx - vx) * (nvx - vx) + (nvy - vy) * (nvy - vy)) + (nvz - vz) * (nvz - vz));
+            double ek = orc_exp(-kapper * dtc);
+            dS[4] = -(3.0 / c->gm) * ac * de * nde / kapper *
+                    ((((ek - 1.0) * (nte - te) - (B + D) * ek) + B * orc_exp(-lambda * dtc)) +
+                     D * orc_exp(-2.0 * lambda * dtc)) / dtc;
+            for (int n = 0; n < 5; n++) {
+                s->Uh0[n * N + k] = s->Uh0[n * N + k] - dtc * dS[n];
+                s->Um0[n * N + k] = s->Um0[n * N + k] + dtc * dS[n];
+            }
+        } else {
+            dS[1] = ac * (um1 * s->Uh[N + k] - uh1 * s->Um[N + k]);
+            dS[2] = ac * (um1 * s->Uh[2 * N + k] - uh1 * s->Um[2 * N + k]);
+            dS[3] = ac * (um1 * s->Uh[3 * N + k] - uh1 * s->Um[3 * N + k]);
+            dS[4] = ac * nde * de * (0.5 * ((nvx * nvx - vx * vx) + (nvy * nvy - vy * vy) + (nvz * nvz - vz * vz))
+                                     + 3.0 / c->gm / 2.0 * (nte - te));
+            for (int n = 0; n < 5; n++) {
+                S_h[n * N + k] = S_h[n * N + k] - dS[n];
+                S_m[n * N + k] = S_m[n * N + k] + dS[n];
+            }
         }
         if (c->flag_ir >= 1) {
             double gr = s->gm_rec[k], gi = s->gm_ion[k];
@@ -918,6 +982,8 @@ void orc_run_solver(orc_state *s) {
     memcpy(s->Um0, s->Um, sizeof(double) * N * nv);
     memcpy(s->Uh0, s->Uh, sizeof(double) * N * 5);
     for (int istep = 1; istep <= c->t_order; istep++) {
+        /* set_dt_coll (:83-93): dt/2 per stage for t_order=2, dt otherwise */
+        s->dt_coll_i = (c->t_order == 2) ? s->dt * 0.5 : s->dt;
         orc_set_coefficients(s, 1);
         orc_set_flux(s, s->Fh, s->Fm);
         orc_source_pip(s, s->Sh, s->Sm);

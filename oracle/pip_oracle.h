@@ -27,11 +27,13 @@ typedef struct {
     int t_order;       /* SSP-RK order, flag_time=1 (2 or 3)                */
     int flag_ir;       /* 0 off, 3 = set_IR case 3                          */
     int flag_ir_type;  /* 0 new (ion_pot/arb_heat), 1 Khomenko, 2 Singh     */
-    int flag_col;      /* parity used by IR formulas (1 = constant alpha)   */
+    int flag_col;      /* 1 constant alpha; 2..7 set_collisional col_type 1..6 */
     int ac_field;      /* 1: per-cell ac supplied by caller (kept as is)    */
     int flag_bnd[4];   /* x-, x+, y-, y+ : 1 periodic, 2..7/11 mirror, 10 free */
     double gm, safety, ro_lim, pr_lim, dx, dy;
     double col, db_clean, t_ir, T0, n0, beta;
+    int flag_pip_imp;      /* 1: implicit collision source (PIP_rot:1333-1358) */
+    double pip_imp_factor; /* its dt limit factor (Scheme_rot.f90:308-309)  */
 } orc_cfg;
 
 typedef struct orc_state orc_state;

@@ -34,8 +34,10 @@ class PIPConfig:
     t_order: int = 2               # SSP-RK order (flag_time=1)
     flag_ir: int = 0               # 0 off, 3 = Popescu empirical rates (PIP_rot.f90:178)
     flag_ir_type: int = 0          # 0 new formulation (ion_pot/arb_heat)
-    flag_col: int = 1              # 1: constant alpha = col
+    flag_col: int = 1              # 1: constant alpha = col; 2..7 set_collisional col_type 1..6
     col: float = 10.0              # collision coefficient alpha_c
+    flag_pip_imp: int = 0          # 1: implicit collision source (PIP_rot.f90:1333-1358)
+    pip_imp_factor: float = 10.0   # its dt factor (Scheme_rot.f90:308-309; setting.txt default)
     ac_field: bool = False         # per-cell alpha_c supplied by the caller
     t_ir: float = 1.0e-5           # setting 't_IR'
     T0: float = 5500.0             # setting 'T_norm'
@@ -55,6 +57,10 @@ class PIPConfig:
                 raise ValueError(f"unsupported boundary flag {f}")
         if self.t_order not in (2, 3):
             raise ValueError("t_order must be 2 or 3 (SSP-RK)")
+        if self.flag_col not in range(1, 8):
+            raise ValueError("flag_col must be 1..7")
+        if self.flag_col >= 2 and self.ac_field:
+            raise ValueError("flag_col 2..7 computes alpha_c; ac_field needs flag_col = 1")
 
     # derived sizes (Model_rot.f90:309-333)
     @property
