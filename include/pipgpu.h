@@ -43,11 +43,13 @@ typedef struct pip_cfg {
     int t_order;       /* SSP-RK order: 2 or 3 (flag_time = 1)             */
     int flag_ir;       /* 0 off, 3 = Popescu et al. empirical rates        */
     int flag_ir_type;  /* 0 (ion_pot/arb_heat), 1, 2                       */
-    int flag_col;      /* 1: constant alpha_c = col (parity used by IR)     */
+    int flag_col;      /* 1 constant alpha_c = col; 2..7 set_collisional   */
     int ac_field;      /* 1: per-cell alpha_c uploaded with the state      */
     int flag_bnd[4];   /* x-, x+, y-, y+: 1 periodic, 2-7/11 mirror, 10 free */
     double gm, safety, ro_lim, pr_lim, dx, dy;
     double col, db_clean, t_ir, T0, n0, beta;
+    int flag_pip_imp;       /* 1: implicit collision source                */
+    double pip_imp_factor;  /* its dt factor (cfl_pn_col)                  */
 } pip_cfg;
 
 typedef struct pip_ctx pip_ctx;
@@ -64,8 +66,8 @@ int pip_upload(pip_ctx *ctx, const double *U_h, const double *U_m, const double 
 int pip_download(pip_ctx *ctx, double *U_h, double *U_m);
 
 /* Prologue tail, Initial_rot.f90:140-160: ghost fill, set_coefficients(.,0)
- * (set_IR incl. arb_heat), cfl(dt); followed by the first pass of the time
- * loop's set_IR/cfl (main.f90:32-36). Returns the first dt. */
+ * (set_collisional for flag_col 2..7, set_IR incl. arb_heat), cfl(dt).
+ * Returns the first dt. */
 int pip_prologue(pip_ctx *ctx, double *dt);
 
 /* set_IR (PIP_rot.f90:127-311, case 3): get_Te rewrites + Gm_ion, Gm_rec. */

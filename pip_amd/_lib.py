@@ -28,6 +28,7 @@ class PipCfg(ctypes.Structure):
         ("pr_lim", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
         ("col", ctypes.c_double), ("db_clean", ctypes.c_double), ("t_ir", ctypes.c_double),
         ("T0", ctypes.c_double), ("n0", ctypes.c_double), ("beta", ctypes.c_double),
+        ("flag_pip_imp", ctypes.c_int), ("pip_imp_factor", ctypes.c_double),
     ]
 
 
@@ -41,6 +42,7 @@ def cfg_to_c(cfg) -> PipCfg:
     c.gm, c.safety, c.ro_lim, c.pr_lim = cfg.gm, cfg.safety, cfg.ro_lim, cfg.pr_lim
     c.dx, c.dy, c.col, c.db_clean = cfg.dx, cfg.dy, cfg.col, cfg.db_clean
     c.t_ir, c.T0, c.n0, c.beta = cfg.t_ir, cfg.T0, cfg.n0, cfg.beta
+    c.flag_pip_imp, c.pip_imp_factor = cfg.flag_pip_imp, cfg.pip_imp_factor
     return c
 
 

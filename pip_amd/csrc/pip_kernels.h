@@ -50,6 +50,8 @@ struct StageArgs {
   const double* U0;   // state at run_solver entry (pointwise; may alias Uout)
   double* Uout;
   const double* ac;       // per-cell alpha_c or nullptr
+  double* ac_out;         // set_collisional col_type >= 1: alpha_c of this stage (nullable)
+  double* U0out;          // implicit collisions: updated stage base for the next stage (nullable)
   const double* arb_heat; // IR type 0
   double* ion_pot_out;    // nullable: ion_pot of this stage
   DevScalars* sc;
@@ -206,6 +208,11 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
       for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + o];
 #pragma unroll
       for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + o];
+      if (col_type(a.ph) >= 1) {  // set_collisional's get_Te_* rewrite E (PIP_rot.f90:57-58)
+        double pr, te;
+        getpr_mhd(um, pr, te, a.ph);
+        getpr_hd(uh, pr, te, a.ph);
+      }
       if (IR) {  // set_IR of set_coefficients(.,1) rewrites E (Solver_rot.f90:111-113)
         double pr, te;
         getpr_hd(uh, pr, te, a.ph);
@@ -264,25 +271,37 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 #pragma unroll
     for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + oc];
     if (a.ac) acc = a.ac[oc];
+    if (col_type(a.ph) >= 1) {  // set_collisional (PIP_rot.f90:44-97), before set_IR
+      acc = set_col_cell(um, uh, a.col, a.ph);
+      if (a.ac_out) {
+        a.ac_out[oc] = acc;
+        for_each_image(ci, cj, ix, (NDIM == 2) ? a.jx : 1, a.bnd,
+                       [&](int xi, int yj, unsigned) { a.ac_out[(long long)yj * ix + xi] = acc; });
+      }
+    }
     double gr = 0.0, gi = 0.0, arb = 0.0;
     if (IR) {
       set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
       if (a.ph.flag_ir_type == 0) arb = a.arb_heat[oc];
     }
-    source_pip(um, uh, acc, gr, gi, arb, a.ph, a.irc.heat_fac, Sh, Sm, ion_pot);
+    // set_dt_coll (Solver_rot.f90:83-93): dt/2 per stage for t_order = 2
+    const double dtc = (a.t_order == 2) ? dt * 0.5 : dt;
+    double dU0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    source_pip(um, uh, acc, gr, gi, arb, a.ph, a.irc.heat_fac, Sh, Sm, ion_pot, dtc, dU0);
     if (a.ion_pot_out) a.ion_pot_out[oc] = ion_pot;
     const double ax = dt_sub / a.dx;
     const int fL = tj * (TX + 1) + ti;
-    if (a.istep == 1) {
 #pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        double u0 = a.U0[n * N + oc];
-        un[n] = u0 - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+    for (int n = 0; n < NVT; n++) {
+      double u0 = a.U0[n * N + oc];
+      if (a.ph.pip_imp == 1) {  // U_m0 += dt_c dS, U_h0 -= dt_c dS (PIP_rot.f90:1355-1356)
+        if (n < 5) u0 = u0 + dU0[n];
+        else if (n >= NV && n < NV + 5) u0 = u0 - dU0[n - NV];
+        if (a.U0out) a.U0out[n * N + oc] = u0;
       }
-    } else {
-#pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        double u0 = a.U0[n * N + oc];
+      if (a.istep == 1) {
+        un[n] = u0 - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+      } else {
         double ur = (n < NV) ? um[n] : uh[n - NV];
         un[n] = c0 * u0 + c1 * ur - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
       }
@@ -398,7 +417,8 @@ struct RoundArgs {
   DevScalars* sc;
   int ix, jx, nx, ny;
   long long N;
-  int do_ir, init_arb, do_cfl, read_rates;
+  int do_ir, init_arb, do_cfl, read_rates, do_col;
+  double* ac_out;  // do_col with col_type >= 1: alpha_c of every cell
   double col, gmin_safety;
   Phys ph;
   IRConst irc;
@@ -423,6 +443,10 @@ __global__ void __launch_bounds__(256) k_round(RoundArgs a) {
     double* um = u;
     double* uh = u + NV;
     double acc = a.ac ? a.ac[k] : a.col;
+    if (a.do_col && col_type(a.ph) >= 1) {  // set_collisional of set_coefficients(.,0)
+      acc = set_col_cell(um, uh, a.col, a.ph);
+      a.ac_out[k] = acc;
+    }
     double gr = 0.0, gi = 0.0;
     if (IR && a.do_ir) {
       set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
@@ -482,8 +506,8 @@ __global__ void __launch_bounds__(256) k_bnd(double* U, int nvt, int ix, int jx,
 // (time += dt, stop_sim, next dt and cmax, activity flags).
 struct FinArgs {
   DevScalars* sc;
-  int mode, into_global, flag_ir, flag_ir_type, flag_divb;
-  double safety, dxyz_min;
+  int mode, into_global, flag_ir, flag_ir_type, flag_divb, pip_imp;
+  double safety, dxyz_min, pip_imp_factor;
 };
 
 __global__ void k_finalize(FinArgs f) {
@@ -492,7 +516,9 @@ __global__ void k_finalize(FinArgs f) {
   if (f.mode == 1 && !sc->active) return;
   if (f.mode == 1) sc->ion_pot_valid = 1;  // source_PIP ran in this step
   double dttemp = dmin(10.0, as_f64(sc->red[RED_DT]));
-  dttemp = dmin(dttemp, dmin(f.safety, 0.3) / as_f64(sc->red[RED_RHOAC]));
+  // cfl_pn_col (Scheme_rot.f90:305-313)
+  if (f.pip_imp == 1) dttemp = dmin(dttemp, f.pip_imp_factor / as_f64(sc->red[RED_RHOAC]));
+  else dttemp = dmin(dttemp, dmin(f.safety, 0.3) / as_f64(sc->red[RED_RHOAC]));
   if (f.flag_ir >= 1) {
     dttemp = dmin(dttemp, dmin(f.safety, 0.1) / dmax(as_f64(sc->red[RED_GMAX]), 1.0e-5));
     if (f.flag_ir_type == 0) {
@@ -658,6 +684,8 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
     double* U1 = e.U1 + M.off;
     StageArgs a;
     a.ac = nullptr;
+    a.ac_out = nullptr;
+    a.U0out = nullptr;
     a.arb_heat = nullptr;
     a.ion_pot_out = nullptr;
     a.sc = nullptr;

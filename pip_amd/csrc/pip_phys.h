@@ -102,10 +102,16 @@ PIP_HD double ppow(double x, double y) {
 struct Phys {
   double gm, ro_lim, pr_lim, safety;
   double T0, n0, beta, t_ir;
-  int flag_col;      // parity selects the IR normalisation
+  int flag_col;      // 1 constant alpha; 2..7 set_collisional (parity selects the IR normalisation)
   int flag_ir;       // 0 or 3
   int flag_ir_type;  // 0, 1, 2
+  int pip_imp;       // 1: implicit collision source (PIP_rot.f90:1333-1358)
 };
+
+constexpr double kPi = 3.1415926535897932;  // Parameters.f90:2
+
+// set_collisional col_type (PIP_rot.f90:21-25): flag_col - 1 for 2..7.
+PIP_HD int col_type(const Phys& p) { return (p.flag_col >= 2 && p.flag_col <= 7) ? p.flag_col - 1 : 0; }
 
 // Stage coefficients: Solver_rot.f90:223-231 (float() conversions kept).
 PIP_HD void stage_coeffs(int istep, int t_order, double dt, double& c0, double& c1,
@@ -490,6 +496,33 @@ PIP_HD void ir_rates(double rho_p, double te_p, const Phys& p, const IRConst& c,
   gi = g / (0.232 + a) / c.rec_fac / c.f_p * p.t_ir;
 }
 
+// set_collisional, one cell, col_type 1..6 (PIP_rot.f90:44-97): get_Te_MHD
+// and get_Te_HD (both rewrite E) and the drift of get_vel_diff
+// (Scheme_rot.f90:988-999); returns alpha_c.
+PIP_HD double set_col_cell(double* um, double* uh, double col, const Phys& p) {
+  const int ct = col_type(p);
+  double pr, te_m, te_h;
+  getpr_mhd(um, pr, te_m, p);
+  getpr_hd(uh, pr, te_h, p);
+  const double tt = te_h + te_m;
+  double a = col * sqrt(tt / 2.0);
+  if (ct == 1) {
+    a = a / sqrt(p.beta / 2.0 * p.gm);
+  } else if (ct >= 3) {
+    const double vd0 = uh[1] / uh[0] - um[1] / um[0];
+    const double vd1 = uh[2] / uh[0] - um[2] / um[0];
+    const double vd2 = uh[3] / uh[0] - um[3] / um[0];
+    const double v2 = (vd0 * vd0 + vd1 * vd1) + vd2 * vd2;
+    a = a * sqrt(1.0 + 9.0 * kPi / 64.0 * p.gm / 2.0 * v2 / tt);
+    if (ct == 3) a = a / sqrt(p.beta / 2.0 * p.gm);
+    else if (ct == 5)
+      a = a / sqrt(p.beta / 2.0 * p.gm) * ppow(p.beta / 2.0 * p.gm / tt / 2.0 + p.gm * kPi / 16.0 * v2, 0.125);
+    else if (ct == 6)
+      a = a / ppow(tt / 2.0 + p.gm * kPi / 16.0 * v2, 0.125);
+  }
+  return a;
+}
+
 // set_IR on one cell: get_Te_HD / get_Te_MHD rewrites, then the rates.
 template <int NV>
 PIP_HD void set_ir_cell(double* um, double* uh, const Phys& p, const IRConst& c, double& gr,
@@ -505,7 +538,8 @@ PIP_HD void set_ir_cell(double* um, double* uh, const Phys& p, const IRConst& c,
 // Rewrites um/uh through cq2pv_* (:1328-1329); writes S_h[5], S_m[5]
 // (S_m(6:nvar) = 0) and ion_pot.
 PIP_HD void source_pip(double* um, double* uh, double ac, double gr, double gi, double arb_heat,
-                       const Phys& p, double heat_fac, double* Sh, double* Sm, double& ion_pot) {
+                       const Phys& p, double heat_fac, double* Sh, double* Sm, double& ion_pot,
+                       double dtc = 0.0, double* dU0 = nullptr) {
   double ph[5], pm[8];
   cq2pv_hd(uh, ph, p);
   cq2pv_mhd(um, pm, p);
@@ -515,16 +549,44 @@ PIP_HD void source_pip(double* um, double* uh, double ac, double gr, double gi, 
   double nte = npr / nde * p.gm * kMuN;
   double dS[5];
   dS[0] = 0.0;
-  dS[1] = ac * (um[0] * uh[1] - uh[0] * um[1]);
-  dS[2] = ac * (um[0] * uh[2] - uh[0] * um[2]);
-  dS[3] = ac * (um[0] * uh[3] - uh[0] * um[3]);
-  dS[4] = ac * nde * de *
-          (0.5 * ((nvx * nvx - vx * vx) + (nvy * nvy - vy * vy) + (nvz * nvz - vz * vz)) +
-           3.0 / p.gm / 2.0 * (nte - te));
+  if (p.pip_imp == 1) {
+    // implicit exchange over dtc = dt_coll_i (:1333-1358): returned as the
+    // increment dU0 = dtc dS of the plasma's stage base (the neutrals' is
+    // -dU0); S gets no collision term
+    const double temp = dtc * (uh[0] + um[0]);
+    const double ex = pexp(-ac * temp);
+    dS[1] = (um[0] * uh[1] - uh[0] * um[1]) * (1.0 - ex) / temp;
+    dS[2] = (um[0] * uh[2] - uh[0] * um[2]) * (1.0 - ex) / temp;
+    dS[3] = (um[0] * uh[3] - uh[0] * um[3]) * (1.0 - ex) / temp;
+    const double lambda = ac * (nde + de);
+    const double kapper = 3.0 * (p.gm - 1.0) * ac * (kMuP * nde + kMuN * de);
+    const double Bq = -1.0 / 3.0 * p.gm * kapper / ((nde + de) * (kapper - lambda)) *
+                      (de * ((nvx - vx) * vx + (nvy - vy) * vy + (nvz - vz) * nvz) +
+                       nde * ((nvx - vx) * nvx + (nvy - vy) * nvy + (nvz - vz) * nvz));
+    const double Dq = -p.gm / 6.0 * kapper * (de - nde) / ((de + nde) * (kapper - 2.0 * lambda)) *
+                      (((nvx - vx) * (nvx - vx) + (nvy - vy) * (nvy - vy)) + (nvz - vz) * (nvz - vz));
+    const double ek = pexp(-kapper * dtc);
+    dS[4] = -(3.0 / p.gm) * ac * de * nde / kapper *
+            ((((ek - 1.0) * (nte - te) - (Bq + Dq) * ek) + Bq * pexp(-lambda * dtc)) +
+             Dq * pexp(-2.0 * lambda * dtc)) / dtc;
 #pragma unroll
-  for (int n = 0; n < 5; n++) {
-    Sh[n] = 0.0 - dS[n];
-    Sm[n] = 0.0 + dS[n];
+    for (int n = 0; n < 5; n++) {
+      dU0[n] = dtc * dS[n];
+      Sh[n] = 0.0;
+      Sm[n] = 0.0;
+    }
+  } else {
+    dS[1] = ac * (um[0] * uh[1] - uh[0] * um[1]);
+    dS[2] = ac * (um[0] * uh[2] - uh[0] * um[2]);
+    dS[3] = ac * (um[0] * uh[3] - uh[0] * um[3]);
+    dS[4] = ac * nde * de *
+            (0.5 * ((nvx * nvx - vx * vx) + (nvy * nvy - vy * vy) + (nvz * nvz - vz * vz)) +
+             3.0 / p.gm / 2.0 * (nte - te));
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      Sh[n] = 0.0 - dS[n];
+      Sm[n] = 0.0 + dS[n];
+    }
   }
   ion_pot = 0.0;
   if (p.flag_ir >= 1) {

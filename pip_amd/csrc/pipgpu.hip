@@ -45,6 +45,7 @@ struct pip_ctx {
   double* gm_ion = nullptr;
   double* arb_heat = nullptr;
   double* ion_pot = nullptr;
+  double* Ub = nullptr;  // implicit collisions: updated stage base
   DevScalars* sc = nullptr;
   hipStream_t stream = nullptr;
   int tx = 16, ty = 15;
@@ -85,7 +86,13 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   a.Uin = Uin;
   a.U0 = U0;
   a.Uout = Uout;
+  a.U0out = nullptr;
+  if (c->cfg.flag_pip_imp == 1) {  // the stage base is updated by every stage
+    a.U0 = (istep == 1) ? U0 : c->Ub;
+    a.U0out = (istep < c->cfg.t_order) ? c->Ub : nullptr;
+  }
   a.ac = c->cfg.ac_field ? c->ac : nullptr;
+  a.ac_out = (col_type(c->ph) >= 1) ? c->ac : nullptr;
   a.arb_heat = c->arb_heat;
   a.ion_pot_out = ion_out;
   a.sc = c->sc;
@@ -153,10 +160,13 @@ static int launch_stage(pip_ctx* c, const StageArgs& a) {
 }
 
 static RoundArgs round_args(pip_ctx* c, int do_ir, int init_arb, int do_cfl, int read_rates,
-                            const double* ion_pot) {
+                            const double* ion_pot, int do_col = 0) {
   RoundArgs r;
   r.U = c->U;
-  r.ac = c->cfg.ac_field ? c->ac : nullptr;
+  const bool col_field = col_type(c->ph) >= 1;
+  r.ac = (c->cfg.ac_field || (col_field && !do_col)) ? c->ac : nullptr;
+  r.do_col = do_col;
+  r.ac_out = col_field ? c->ac : nullptr;
   r.gm_rec = c->gm_rec;
   r.gm_ion = c->gm_ion;
   r.arb_heat = c->arb_heat;
@@ -202,6 +212,8 @@ static int launch_finalize(pip_ctx* c, int mode, int into_global) {
   f.flag_divb = c->cfg.flag_divb;
   f.safety = c->cfg.safety;
   f.dxyz_min = c->dxyz_min;
+  f.pip_imp = c->cfg.flag_pip_imp;
+  f.pip_imp_factor = c->cfg.pip_imp_factor;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, c->stream, f);
   HIPCHK(hipGetLastError());
   return PIP_OK;
@@ -247,6 +259,11 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
   if (cfg->flag_ir != 0 && cfg->flag_ir != 3)
     return fail(PIP_ERR_UNSUPPORTED, "flag_IR must be 0 or 3 (set_IR case 3)");
   if (cfg->flag_ir_type < 0 || cfg->flag_ir_type > 2) return fail(PIP_ERR_ARG, "flag_IR_type in 0..2");
+  if (cfg->flag_col < 1 || cfg->flag_col > 7) return fail(PIP_ERR_UNSUPPORTED, "flag_col must be 1..7");
+  if (cfg->flag_col >= 2 && cfg->ac_field)
+    return fail(PIP_ERR_ARG, "flag_col 2..7 computes alpha_c; ac_field needs flag_col = 1");
+  if (cfg->flag_pip_imp != 0 && cfg->flag_pip_imp != 1) return fail(PIP_ERR_ARG, "flag_pip_imp must be 0 or 1");
+  if (cfg->flag_pip_imp == 1 && !(cfg->pip_imp_factor > 0)) return fail(PIP_ERR_ARG, "pip_imp_factor must be > 0");
   for (int s = 0; s < 2 * cfg->ndim; s++) {
     int t = bnd_kind(cfg->flag_bnd[s]);
     if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
@@ -270,6 +287,7 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
   c->ph.flag_col = cfg->flag_col;
   c->ph.flag_ir = cfg->flag_ir;
   c->ph.flag_ir_type = cfg->flag_ir_type;
+  c->ph.pip_imp = cfg->flag_pip_imp;
   c->irc = ir_constants(c->ph);
   double dy = (cfg->ndim >= 2) ? cfg->dy : 100.0, dz = 100.0;  // Model_rot.f90:424-429
   c->dxyz_min = dmin(dmin(cfg->dx, dy), dz);
@@ -287,7 +305,9 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
   if ((e = hipMalloc(&c->U, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U");
   if ((e = hipMalloc(&c->U1, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U1");
   if (cfg->t_order >= 3 && (e = hipMalloc(&c->U2, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U2");
-  if (cfg->ac_field && (e = hipMalloc(&c->ac, bytes)) != hipSuccess) return cleanup("hipMalloc ac");
+  if ((cfg->ac_field || col_type(c->ph) >= 1) && (e = hipMalloc(&c->ac, bytes)) != hipSuccess)
+    return cleanup("hipMalloc ac");
+  if (cfg->flag_pip_imp == 1 && (e = hipMalloc(&c->Ub, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc Ub");
   if (cfg->flag_ir) {
     if ((e = hipMalloc(&c->gm_rec, bytes)) != hipSuccess) return cleanup("hipMalloc gm_rec");
     if ((e = hipMalloc(&c->gm_ion, bytes)) != hipSuccess) return cleanup("hipMalloc gm_ion");
@@ -316,6 +336,7 @@ int pip_destroy(pip_ctx* c) {
   (void)hipFree(c->gm_ion);
   (void)hipFree(c->arb_heat);
   (void)hipFree(c->ion_pot);
+  (void)hipFree(c->Ub);
   (void)hipFree(c->sc);
   (void)hipFree(c->adjV);
   (void)hipFree(c->adjP);
@@ -384,7 +405,9 @@ int pip_prologue(pip_ctx* c, double* dt) {
   HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
   c->arb_done = false;
   TRY(launch_bnd(c, c->U));        // boundary map of the initial state
-  TRY(pip_set_ir(c));              // set_coefficients(U_h,U_m,0) (Initial_rot.f90:140)
+  // set_coefficients(U_h,U_m,0) (Initial_rot.f90:140): set_collisional, set_IR
+  if (col_type(c->ph) >= 1) TRY(launch_round(c, round_args(c, 0, 0, 0, 0, nullptr, 1)));
+  TRY(pip_set_ir(c));
   TRY(pip_cfl(c, dt, 1));          // cfl(U_h,U_m,dt) (Initial_rot.f90:152)
   c->top_pending = true;
   return PIP_OK;
@@ -595,6 +618,8 @@ static dim3 cell_grid(const pip_ctx* c) { return dim3((unsigned)((c->N + 255) / 
 static int adj_check(const pip_ctx* c) {
   if (c->cfg.ndim != 2 || c->nv != ADJ_NV) return fail(PIP_ERR_UNSUPPORTED, "adjoint needs ndim = 2 and flag_divb = 1");
   if (c->cfg.flag_ir != 0) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers flag_IR = 0");
+  if (c->cfg.flag_col != 1 || c->cfg.flag_pip_imp != 0)
+    return fail(PIP_ERR_UNSUPPORTED, "adjoint covers flag_col = 1 (constant or per-cell alpha) and explicit collisions");
   if (c->cfg.t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers SSP-RK2 (t_order = 2)");
   for (int s = 0; s < 4; s++)
     if (c->bnd.type[s] == 10) return fail(PIP_ERR_UNSUPPORTED, "adjoint covers periodic and mirror boundaries");
@@ -809,7 +834,8 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   if (!base || !members || !out || n <= 0) return fail(PIP_ERR_ARG, "null argument or n <= 0");
   *out = nullptr;
   if (base->ndim != 1 || base->nvar_m != 8) return fail(PIP_ERR_UNSUPPORTED, "ensemble members are 1-D (nvar_m = 8)");
-  if (base->flag_ir != 0 || base->ac_field) return fail(PIP_ERR_UNSUPPORTED, "ensemble: flag_IR = 0, constant alpha_c");
+  if (base->flag_ir != 0 || base->ac_field || base->flag_col != 1 || base->flag_pip_imp != 0)
+    return fail(PIP_ERR_UNSUPPORTED, "ensemble: flag_IR = 0, constant alpha_c, explicit collisions");
   if (base->t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "ensemble: t_order = 2");
   for (int s = 0; s < 2; s++) {
     int t = bnd_kind(base->flag_bnd[s]);

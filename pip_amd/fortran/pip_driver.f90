@@ -69,6 +69,7 @@ program pip_driver
   cfg%gm = gm; cfg%safety = 0.3d0; cfg%ro_lim = 1.0d-8; cfg%pr_lim = 1.0d-8
   cfg%dx = dx; cfg%dy = 100.0d0; cfg%col = 100.0d0; cfg%db_clean = 0.7d0
   cfg%t_ir = 1.0d-5; cfg%T0 = 5500.0d0; cfg%n0 = 7.5d16; cfg%beta = 0.1d0
+  cfg%flag_pip_imp = 0; cfg%pip_imp_factor = 10.0d0
 
   call pip_check(pip_create(cfg, ctx), 'pip_create')
   call pip_check(pip_upload(ctx, U_h, U_m, c_null_ptr), 'pip_upload')

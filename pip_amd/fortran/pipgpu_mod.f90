@@ -19,6 +19,8 @@ module pipgpu
     integer(c_int) :: flag_bnd(4)
     real(c_double) :: gm, safety, ro_lim, pr_lim, dx, dy
     real(c_double) :: col, db_clean, t_ir, T0, n0, beta
+    integer(c_int) :: flag_pip_imp
+    real(c_double) :: pip_imp_factor
   end type pip_cfg
 
   interface

@@ -19,7 +19,9 @@ def test_library_exports_every_header_symbol():
 
 @pytest.mark.parametrize("bad", [
     dict(ndim=3), dict(nvar_m=9), dict(t_order=4), dict(flag_ir=1), dict(ix=6),
-    dict(flag_bnd=(0, 1, 1, 1)), dict(dx=0.0),
+    dict(flag_bnd=(0, 1, 1, 1)), dict(dx=0.0), dict(flag_col=8), dict(flag_col=0),
+    dict(flag_col=3, ac_field=1), dict(flag_pip_imp=2), dict(flag_pip_imp=1, pip_imp_factor=0.0),
+    dict(ix=40000),
 ])
 def test_create_rejects_invalid_config(bad):
     cfg = PIPConfig(ndim=1, nx=64)

@@ -198,3 +198,37 @@ def test_stop_sim_nan_and_small_dt():
     s = PIPSolver(cfg2, uh, um, ac)
     n, _ = s.advance(nsteps=10)
     assert n == 1 and (s.status() & 1)
+
+
+@pytest.mark.parametrize("flag_col", [2, 3, 4, 5, 6, 7])
+def test_temperature_dependent_collisions(flag_col):
+    """set_collisional col_type 1..6 (PIP_rot.f90:44-97): alpha_c from T_n,
+    T_p and the drift, recomputed every stage, with its get_Te rewrites."""
+    say(f"case orszag_tang n=32 flag_col={flag_col} steps=30")
+    cfg, uh, um, ac = problems.orszag_tang(32)
+    cfg = cfg.replace(flag_col=flag_col, col=30.0)
+    o, s, n, _ = run_pair(cfg, uh, um, None, nsteps=30)
+    assert n == 30
+    compare(o, s, cfg, 0.0)
+
+
+@pytest.mark.parametrize("case", ["brio_wu", "orszag_tang", "kh_ir_col3"])
+def test_implicit_collisions(case):
+    """flag_pip_imp = 1 (PIP_rot.f90:1333-1358, Scheme_rot.f90:308-309): the
+    exchange integrated over dt/2 per stage and applied to the stage base."""
+    say(f"case implicit collisions {case}")
+    if case == "brio_wu":
+        cfg, uh, um, ac = problems.brio_wu(512, col=1.0e4)
+        nsteps = 200
+    elif case == "orszag_tang":
+        cfg, uh, um, ac = problems.orszag_tang(32)
+        nsteps = 40
+    else:
+        cfg, uh, um, ac = problems.kelvin_helmholtz(32)
+        cfg = cfg.replace(flag_col=3, col=100.0)
+        nsteps = 30
+    cfg = cfg.replace(flag_pip_imp=1)
+    o, s, n, _ = run_pair(cfg, uh, um, None if cfg.flag_col != 1 else ac, nsteps=nsteps)
+    assert n == nsteps
+    assert s.scalars()["dt"] == o.scalars()["dt"]
+    compare(o, s, cfg, 0.0)
