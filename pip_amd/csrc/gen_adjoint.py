@@ -165,14 +165,19 @@ class Gen:
                     continue
                 out.append(f"{pad}a_{x} += {ab} * ({cc(d)});")
 
-    def _emit(self, blk, out, ind):
+    def _emit(self, blk, out, ind, recompute=False):
         """forward of blk, then its branches (each fully emitted), then the
-        reverse of blk's own statements."""
+        reverse of blk's own statements.  With recompute, blk's forward is
+        evaluated again (from inputs passed through an empty asm barrier, so
+        the compiler cannot reuse the first values) before its reverse: its
+        intermediates need not stay live across the branches."""
         pad = " " * ind
         # forward statements before the first branch, branches, reverse
         self._fwd(blk, out, ind)
+        has_branch = False
         for it in blk.items:
             if it[0] == "if":
+                has_branch = True
                 first = True
                 for cond, child in it[1]:
                     if cond is None:
@@ -184,12 +189,17 @@ class Gen:
                     first = False
                     self._emit(child, out, ind + 2)
                 out.append(f"{pad}}}")
+        if recompute and has_branch:
+            out.append(f"{pad}// recompute the side quantities for their reverse")
+            for x in self.inputs:
+                out.append(f"{pad}PIP_OPAQUE({x});")
+            self._fwd(blk, out, ind)
         self._rev_stmts(blk, out, ind)
 
     def all_syms(self):
         return list(self.syms.values())
 
-    def emit(self, signature, prologue, epilogue, out_seeds):
+    def emit(self, signature, prologue, epilogue, out_seeds, recompute_root=False):
         """out_seeds: list of (symbol name, C expression of its adjoint)."""
         body = []
         inter = [s for s in self.all_syms() if s not in self.inputs]
@@ -202,7 +212,7 @@ class Gen:
             body.append("  double " + ", ".join(f"a_{s} = 0.0" for s in allv[i:i + 6]) + ";")
         for name, expr in out_seeds:
             body.append(f"  a_{name} = {expr};")
-        self._emit(self.root, body, 2)
+        self._emit(self.root, body, 2, recompute=recompute_root)
         return signature + " {\n" + prologue + "\n".join(body) + "\n" + epilogue + "}\n"
 
 
@@ -382,10 +392,10 @@ def gen_hlld():
         glm(blk, outs)
     sig = ("PIP_HD void hlld9_flux_vjp(const double* vl, const double* vr, double gm, double ch, "
            "const double* lam, double* lvl, double* lvr)")
-    pro = "".join(f"  const double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(9))
+    pro = "".join(f"  double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(9))
     epi = "".join(f"  lvl[{k}] += a_vl{k};\n  lvr[{k}] += a_vr{k};\n" for k in range(9))
     seeds = [(f"F{k}", f"lam[{k}]") for k in range(9)]
-    return g.emit(sig, pro, epi, seeds)
+    return g.emit(sig, pro, epi, seeds, recompute_root=RECOMPUTE)
 
 
 # ================================================================= source
@@ -424,6 +434,9 @@ def gen_source():
     return g.emit(sig, pro, epi, seeds)
 
 
+RECOMPUTE = os.environ.get("PIP_ADJ_RECOMPUTE", "1") == "1"
+
+
 def main():
     parts = [
         "// pip_adj_gen.h -- GENERATED by gen_adjoint.py; do not edit by hand.\n"
@@ -431,7 +444,15 @@ def main():
         "// HLLD flux with HLLC-G fallback and GLM psi terms (HLL_rot.f90:691-910)\n"
         "// and the collision source (PIP_rot.f90:1359-1368), at the branch taken\n"
         "// by the forward point.  Each function accumulates into its outputs.\n"
-        "#pragma once\n#include \"pip_phys.h\"\n\nnamespace pip {\n\n",
+        "#pragma once\n#include \"pip_phys.h\"\n\n"
+        "// Empty asm barrier: the compiler must treat x as a new value, so a\n"
+        "// recomputation from it is not merged with the first evaluation.\n"
+        "#if defined(__HIP_DEVICE_COMPILE__)\n"
+        "#define PIP_OPAQUE(x) __asm__ volatile(\"\" : \"+v\"(x))\n"
+        "#else\n"
+        "#define PIP_OPAQUE(x) __asm__ volatile(\"\" : \"+x\"(x))\n"
+        "#endif\n\n"
+        "namespace pip {\n\n",
         gen_hllc(), "\n", gen_hlld(), "\n", gen_source(), "\n}  // namespace pip\n",
     ]
     path = os.path.join(HERE, "pip_adj_gen.h")

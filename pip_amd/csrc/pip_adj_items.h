@@ -35,12 +35,14 @@ namespace pip {
 struct AdjArgs {
   const double* Uin;      // stage input (nvt planes, ghosts filled)
   const double* lam_out;  // adjoint of the stage output
-  double* lam_in;         // += adjoint of Uin
-  double* lam_U0;         // += adjoint of U0 (may alias lam_in for s = 1)
+  double* lam_in;         // adjoint of Uin (assigned here, completed by the gather)
+  double* lam_U0;         // adjoint of U0 (s = 2: assigned c0 mu; aliases lam_in for s = 1)
+  const double* lam_base; // s = 1: the stage-2 contribution to lam_U0 (nullable)
   double* lam_ac;         // += adjoint of alpha_c (nullable)
   const double* ac;       // per-cell alpha_c (nullable: constant col)
   double* V;              // nvt planes: primitives of Uin (all cells)
-  double* P;              // 2 * 4 * nvt planes: per-face contributions
+  double* P;              // 2 dirs x 2 sides x nvt planes: face-state adjoints,
+                          // then 2 dirs x 2 fluids planes of packed routing codes
   const double* dtc;      // device: {dt, cmax} of the forward step
   int ix, jx, nx, ny;
   long long N;
@@ -91,20 +93,27 @@ PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
     for (int n = 0; n < 5; n++) a.V[(NV + n) * N + k] = vh[n];
   }
   const int i = (int)(k % a.ix), j = (int)(k / a.ix);
-  if (!adj_interior(a, i, j)) return;
+  if (!adj_interior(a, i, j)) {
+    // the adjoint lives on interior cells; stage 1 writes the step's result
+    if (a.istep == 1) {
+#pragma unroll
+      for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = 0.0;
+    }
+    return;
+  }
   double c0, c1, dt_sub, psi_fac;
   adj_coeffs(a, c0, c1, dt_sub, psi_fac);
   double mu[NVT];
 #pragma unroll
   for (int n = 0; n < NVT; n++) mu[n] = adj_mu(a, n, k, psi_fac);
-  if (a.istep == 1) {
+  if (a.istep == 1) {  // lam_in == lam_U0: U_n enters both as U0 and as Uin
 #pragma unroll
-    for (int n = 0; n < NVT; n++) a.lam_U0[n * N + k] += mu[n];
+    for (int n = 0; n < NVT; n++) a.lam_U0[n * N + k] = (a.lam_base ? a.lam_base[n * N + k] : 0.0) + mu[n];
   } else {
 #pragma unroll
     for (int n = 0; n < NVT; n++) {
-      a.lam_U0[n * N + k] += c0 * mu[n];
-      a.lam_in[n * N + k] += c1 * mu[n];
+      a.lam_U0[n * N + k] = c0 * mu[n];
+      a.lam_in[n * N + k] = c1 * mu[n];
     }
   }
   // S_m[0..4] = +dS, S_h[0..4] = -dS (PIP_rot.f90:1366-1367)
@@ -134,10 +143,18 @@ PIP_HD int adj_perm(int n, int dir) {
   return n;
 }
 
+// Packed routing codes of the faces of one direction and fluid: bits
+// 4k..4k+1 the left-state branch, 4k+2..4k+3 the right-state branch of the
+// fluid's k-th variable (recon_code), at the face's + cell.
+PIP_HD unsigned long long* adj_codes(const AdjArgs& a, int dir, int fl) {
+  return reinterpret_cast<unsigned long long*>(a.P + (4LL * ADJ_NVT + 2 * dir + fl) * a.N);
+}
+
 // One thread per face and fluid (FL 0: plasma, HLLD with GLM; FL 1:
 // neutrals, HLLC); R = the cell on the + side of the face.
 // dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
-// Writes P[(dir*4 + q) * NVT + n][R] = contribution to cell R + (q-2) st.
+// Writes P[(dir*2 + side) * NVT + n][R]: the adjoints of the left (side 0)
+// and right (side 1) face states of variable n, density floor applied.
 template <int FL>
 PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
@@ -180,50 +197,59 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   else hllc_flux_vjp(vl, vr, a.ph.gm, lamF, lvl, lvr);
   if (flL) lvl[0] = 0.0;
   if (flR) lvr[0] = 0.0;
-  double* P = a.P + (long long)dir * 4 * NVT * N;
+  double* P = a.P + (long long)dir * 2 * NVT * N;
+  unsigned long long codes = 0ull;
 #pragma unroll
   for (int n = 0; n < NF; n++) {
     const int s = adj_perm(OFF + n, dir);
+    P[s * N + oR] = lvl[n];
+    P[(NVT + s) * N + oR] = lvr[n];
     const double* p = a.V + s * N + oR;
-    const double v0 = p[-2 * st], v1 = p[-st], v2 = p[0], v3 = p[st];
-    double gl0, gl1, gl2, gr0, gr1, gr2;
-    recon_left_grad(v0, v1, v2, gl0, gl1, gl2);
-    recon_right_grad(v1, v2, v3, gr0, gr1, gr2);
-    P[(0 * NVT + s) * N + oR] = lvl[n] * gl0;
-    P[(1 * NVT + s) * N + oR] = lvl[n] * gl1 + lvr[n] * gr0;
-    P[(2 * NVT + s) * N + oR] = lvl[n] * gl2 + lvr[n] * gr1;
-    P[(3 * NVT + s) * N + oR] = lvr[n] * gr2;
+    const unsigned cl = recon_code(p[-2 * st], p[-st], p[0]);
+    const unsigned cr = recon_code(p[-st], p[0], p[st]);
+    codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
   }
+  adj_codes(a, dir, FL)[oR] = codes;
 }
 
 // Sum of the face contributions to the primitives of the cell at (i, j),
-// with the sign map `mask` of a ghost image applied.
+// with the sign map `mask` of a ghost image applied.  A face with + cell r
+// (stride st) takes V_L from cells r-2, r-1, r and V_R from r-1, r, r+1,
+// with the weights of the recorded minmod branches.
 PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double* lv) {
-  constexpr int NVT = ADJ_NVT;
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
   const long long N = a.N;
   const long long o = (long long)j * a.ix + i;
-  const double* Px = a.P;
-  const double* Py = a.P + 4LL * NVT * N;
-  const bool rowx = (j >= 2 && j <= a.ny + 1);
-  const bool colx = (i >= 2 && i <= a.nx + 1);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int ri = i + 2 - q;  // x face with R at (ri, j)
-    if (rowx && ri >= 2 && ri <= a.nx + 2) {
-      const long long r = o + (2 - q);
+  for (int dir = 0; dir < 2; dir++) {
+    const int c = dir ? j : i;  // coordinate along the sweep
+    const int hi = dir ? a.ny + 2 : a.nx + 2;  // faces have + cells 2..hi
+    const bool row_ok = dir ? (i >= 2 && i <= a.nx + 1) : (j >= 2 && j <= a.ny + 1);
+    if (!row_ok) continue;
+    const long long st = dir ? a.ix : 1;
+    const double* PL = a.P + (long long)dir * 2 * NVT * N;
+    const double* PR = PL + (long long)NVT * N;
+    const bool f2 = c + 2 <= hi, f1 = c + 1 >= 2 && c + 1 <= hi, f0 = c >= 2 && c <= hi, fm = c - 1 >= 2;
 #pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        const double v = Px[(q * NVT + n) * N + r];
-        lv[n] += ((mask >> n) & 1u) ? -v : v;
-      }
-    }
-    const int rj = j + 2 - q;  // y face with R at (i, rj)
-    if (colx && rj >= 2 && rj <= a.ny + 2) {
-      const long long r = o + (long long)(2 - q) * a.ix;
-#pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        const double v = Py[(q * NVT + n) * N + r];
-        lv[n] += ((mask >> n) & 1u) ? -v : v;
+    for (int fl = 0; fl < 2; fl++) {
+      const unsigned long long* C = adj_codes(a, dir, fl);
+      const unsigned long long k2 = f2 ? C[o + 2 * st] : 0ull, k1 = f1 ? C[o + st] : 0ull,
+                               k0 = f0 ? C[o] : 0ull, km = fm ? C[o - st] : 0ull;
+      const int n0 = fl ? NV : 0, n1 = fl ? NVT : NV;
+      for (int n = n0; n < n1; n++) {
+        const int sh = 4 * (n - n0);
+        double acc = 0.0;
+        if (f2) acc += PL[n * N + o + 2 * st] * recon_w((unsigned)(k2 >> sh) & 3u, false, 0);
+        if (f1) {
+          acc += PL[n * N + o + st] * recon_w((unsigned)(k1 >> sh) & 3u, false, 1);
+          acc += PR[n * N + o + st] * recon_w((unsigned)(k1 >> (sh + 2)) & 3u, true, 0);
+        }
+        if (f0) {
+          acc += PL[n * N + o] * recon_w((unsigned)(k0 >> sh) & 3u, false, 2);
+          acc += PR[n * N + o] * recon_w((unsigned)(k0 >> (sh + 2)) & 3u, true, 1);
+        }
+        if (fm) acc += PR[n * N + o - st] * recon_w((unsigned)(km >> (sh + 2)) & 3u, true, 2);
+        lv[n] += ((mask >> n) & 1u) ? -acc : acc;
       }
     }
   }

@@ -32,6 +32,30 @@ PIP_HD void recon_right_grad(double v0, double v1, double v2, double& g0, double
   }
 }
 
+// Branch of minmod(v0, v1, v2) taken at the forward point: 0 zero slope,
+// 1 slope X = v2 - v1, 2 slope Y = v1 - v0 (the routing of both face states).
+PIP_HD unsigned recon_code(double v0, double v1, double v2) {
+  const double X = v2 - v1, Y = v1 - v0;
+  const double SX = copysign(1.0, X);
+  const double m = dmin(fabs(X), SX * Y);
+  if (!(m > 0.0)) return 0u;
+  return (fabs(X) < SX * Y) ? 1u : 2u;
+}
+
+// Weights of the three cells in V_L = v1 + mm/2 (left) and V_R = v1 - mm/2
+// (right) for a branch code, as d/d(v0, v1, v2).
+PIP_HD double recon_w(unsigned code, bool right, int q) {
+  if (code == 0u) return (q == 1) ? 1.0 : 0.0;
+  if (code == 1u) {  // slope v2 - v1
+    if (q == 0) return 0.0;
+    if (q == 1) return right ? 1.5 : 0.5;
+    return right ? -0.5 : 0.5;
+  }
+  if (q == 2) return 0.0;  // slope v1 - v0
+  if (q == 1) return right ? 0.5 : 1.5;
+  return right ? 0.5 : -0.5;
+}
+
 // Transpose of u2v_mhd (HLL_rot.f90:1159-1186) at u: lu += (dV/dU)^T lv,
 // with the density and pressure floors frozen as at u.
 template <int NV>

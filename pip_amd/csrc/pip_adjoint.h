@@ -30,10 +30,6 @@ __global__ void __launch_bounds__(256) k_zero_ghosts(double* x, int nvt, int ix,
   zero_ghosts_item(x, nvt, ix, jx, N, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ void __launch_bounds__(256) k_fill(double* x, long long n, double v) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) x[k] = v;
-}
 
 // Step bookkeeping for recorded / replayed runs: hist[2k] = dt, hist[2k+1]
 // = cmax of step k.

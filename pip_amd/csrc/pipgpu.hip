@@ -58,7 +58,7 @@ struct pip_ctx {
   int device = 0;
   // adjoint work buffers (allocated on the first pip_adj_step)
   double* adjV = nullptr;   // nvt planes
-  double* adjP = nullptr;   // 8 * nvt planes
+  double* adjP = nullptr;   // 4 * nvt planes (2 dirs x 2 face sides) + 4 planes of routing codes
   double* lamB = nullptr;   // nvt planes
   double* lamC = nullptr;   // nvt planes
 };
@@ -630,7 +630,7 @@ static int adj_alloc(pip_ctx* c) {
   if (c->adjV) return PIP_OK;
   const size_t pb = sizeof(double) * (size_t)c->N * c->nvt;
   HIPCHK(hipMalloc(&c->adjV, pb));
-  HIPCHK(hipMalloc(&c->adjP, 8 * pb));
+  HIPCHK(hipMalloc(&c->adjP, 4 * pb + 4 * sizeof(double) * (size_t)c->N));
   HIPCHK(hipMalloc(&c->lamB, pb));
   HIPCHK(hipMalloc(&c->lamC, pb));
   return PIP_OK;
@@ -643,6 +643,7 @@ static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* 
   a.lam_out = lam_out;
   a.lam_in = lam_in;
   a.lam_U0 = lam_U0;
+  a.lam_base = nullptr;
   a.lam_ac = lam_ac;
   a.ac = c->cfg.ac_field ? c->ac : nullptr;
   a.V = c->adjV;
@@ -678,11 +679,6 @@ static int adj_stage(pip_ctx* c, const AdjArgs& a) {
   return PIP_OK;
 }
 
-static int fill(pip_ctx* c, double* x, long long n) {
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, x, n, 0.0);
-  HIPCHK(hipGetLastError());
-  return PIP_OK;
-}
 
 extern "C" {
 
@@ -726,6 +722,23 @@ int pip_replay_steps(pip_ctx* c, int n, const double* hist, double* Un, double* 
   const size_t sb = sizeof(double) * (size_t)c->N * c->nvt;
   TRY(set_loop(c, 1.0e300, (long long)1 << 62, 1, -1));
   c->top_pending = false;
+  if (Un && U1 && c->cfg.t_order == 2 && c->cfg.flag_pip_imp == 0 && n > 0) {
+    // stages write straight into the save buffers: step k reads Un[k],
+    // writes U1[k] and then Un[k+1] (the last step writes the state)
+    const size_t S = (size_t)c->N * c->nvt;
+    HIPCHK(hipMemcpyAsync(Un, c->U, sb, hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 0; k < n; k++) {
+      hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
+      double* un = Un + (size_t)k * S;
+      double* u1 = U1 + (size_t)k * S;
+      double* out = (k + 1 < n) ? Un + (size_t)(k + 1) * S : c->U;
+      TRY(launch_stage(c, stage_args(c, 1, un, un, u1, 0, nullptr)));
+      TRY(launch_stage(c, stage_args(c, 2, u1, un, out, 2, nullptr)));
+      TRY(launch_finalize(c, 1, 0));
+    }
+    HIPCHK(hipGetLastError());
+    return PIP_OK;
+  }
   for (int k = 0; k < n; k++) {
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
     if (Un) HIPCHK(hipMemcpyAsync(Un + (size_t)k * c->N * c->nvt, c->U, sb, hipMemcpyDeviceToDevice, c->stream));
@@ -750,14 +763,13 @@ int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* d
   if (!c || !Un || !U1 || !dtc || !lam) return fail(PIP_ERR_ARG, "null argument");
   TRY(adj_check(c));
   TRY(adj_alloc(c));
-  const long long nn = c->N * c->nvt;
-  TRY(fill(c, c->lamB, nn));
-  TRY(fill(c, c->lamC, nn));
-  // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1)
+  // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1):
+  // lam -> lamB (adjoint of U1) and lamC (c0 part of the adjoint of U_n)
   TRY(adj_stage(c, adj_args(c, 2, U1, lam, c->lamB, c->lamC, lam_ac, dtc)));
-  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n)
-  TRY(adj_stage(c, adj_args(c, 1, Un, c->lamB, c->lamC, c->lamC, lam_ac, dtc)));
-  HIPCHK(hipMemcpyAsync(lam, c->lamC, sizeof(double) * (size_t)nn, hipMemcpyDeviceToDevice, c->stream));
+  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB (+ lamC) -> lam
+  AdjArgs a1 = adj_args(c, 1, Un, c->lamB, lam, lam, lam_ac, dtc);
+  a1.lam_base = c->lamC;
+  TRY(adj_stage(c, a1));
   return PIP_OK;
 }
 
