@@ -2,8 +2,9 @@
 face fluxes and the coupling source of the two-fluid step.
 
 The forward computations are restated here as SSA statements (one
-assignment per intermediate) organised in a tree of branches that mirrors the
-reference (HLL_rot.f90:615-687 HLLC, :691-910 HLLD with the HLLC-G fallback
+assignment per intermediate; mathematically the reference's formulas, with
+shared reciprocals and primitive-variable forms where they save divisions)
+organised in a tree of branches that mirrors the reference (HLL_rot.f90:615-687 HLLC, :691-910 HLLD with the HLLC-G fallback
 and the GLM psi flux, PIP_rot.f90:1359-1368 collision source).  For every
 statement y = f(x1..xk) the generator emits the local chain rule
 x_i_bar += y_bar * df/dx_i (derivatives by sympy), in reverse order, after
@@ -73,14 +74,16 @@ class Block:
         self.items = []  # ('let', sym, expr, kind) | ('if', [(cond, Block), ...])
 
     def let(self, name, expr, kind="expr"):
+        e = self.g.lower(self, sp.sympify(expr)) if kind == "expr" else sp.sympify(expr)
         s = self.g.sym(name)
-        self.items.append(("let", s, sp.sympify(expr), kind))
+        self.items.append(("let", s, e, kind))
         return s
 
     def out(self, name, expr):
         """Assign an output symbol (may be assigned in several branches)."""
+        e = self.g.lower(self, sp.sympify(expr))
         s = self.g.syms.get(name) or self.g.sym(name)
-        self.items.append(("let", s, sp.sympify(expr), "expr"))
+        self.items.append(("let", s, e, "expr"))
         return s
 
     def minv(self, name, a, b):
@@ -88,6 +91,13 @@ class Block:
 
     def maxv(self, name, a, b):
         return self.let(name, sp.Max(a, b, evaluate=False), "max")
+
+    def select(self, name, cond, a, b):
+        """name = cond ? a : b for symbols/constants a, b; the reverse routes
+        the adjoint to the taken operand without a branch."""
+        s = self.g.sym(name)
+        self.items.append(("let", s, (cond, sp.sympify(a), sp.sympify(b)), "sel"))
+        return s
 
     def const(self, name, code):
         """A value with zero derivative (e.g. sign), given as C code."""
@@ -121,6 +131,45 @@ class Gen:
         self.inputs.append(s)
         return s
 
+    # ------------------------------------------------------------ lowering
+    # Every quotient and square root becomes its own statement, so that its
+    # reverse costs one division (q = t_bar / d; n_bar += q; d_bar -= q t)
+    # instead of symbolic derivatives that divide again by d or d^2; the
+    # remaining expressions are polynomials in statement symbols.
+    def _temp(self, blk, expr, kind):
+        self._ntemp = getattr(self, "_ntemp", 0) + 1
+        t = self.sym(f"{self.name}{self._ntemp}")
+        blk.items.append(("let", t, expr, kind))
+        return t
+
+    def _atom(self, blk, e):
+        return e if (e.is_Symbol or e.is_Number) else self._temp(blk, e, "expr")
+
+    def lower(self, blk, e):
+        if e.is_Atom:
+            return e
+        if isinstance(e, sp.Pow):
+            b, x = e.args
+            if x == sp.Rational(1, 2):
+                return self._temp(blk, self._atom(blk, self.lower(blk, b)), "sqrt")
+            if x.is_Rational and x < 0:
+                d = self.lower(blk, sp.Pow(b, -x))
+                return self._temp(blk, (sp.Integer(1), self._atom(blk, d)), "div")
+            return sp.Pow(self.lower(blk, b), x)
+        if isinstance(e, sp.Mul):
+            num, den = [], []
+            for f in e.args:
+                if isinstance(f, sp.Pow) and f.args[1].is_Rational and f.args[1] < 0:
+                    den.append(sp.Pow(f.args[0], -f.args[1]))
+                else:
+                    num.append(f)
+            n = sp.Mul(*[self.lower(blk, f) for f in num])
+            if not den:
+                return n
+            d = self.lower(blk, sp.Mul(*den))
+            return self._temp(blk, (self._atom(blk, n), self._atom(blk, d)), "div")
+        return e.func(*[self.lower(blk, a) for a in e.args])
+
     # ------------------------------------------------------------ emission
     def _fwd(self, blk, out, ind):
         pad = " " * ind
@@ -135,6 +184,12 @@ class Gen:
                 elif kind == "max":
                     a, b = e.args
                     out.append(f"{pad}{s} = ({cc(a)} > {cc(b)}) ? {cc(a)} : {cc(b)};")
+                elif kind == "sel":
+                    out.append(f"{pad}{s} = ({e[0]}) ? {cc(e[1])} : {cc(e[2])};")
+                elif kind == "div":
+                    out.append(f"{pad}{s} = {cc(e[0])} / {cc(e[1])};")
+                elif kind == "sqrt":
+                    out.append(f"{pad}{s} = sqrt({cc(e)});")
                 else:
                     out.append(f"{pad}{s} = {cc(e)};")
             else:
@@ -149,6 +204,29 @@ class Gen:
             if kind == "const":
                 continue
             ab = f"a_{s}"
+            if kind == "sel":
+                known = set(self.syms.values())
+                cond, x, y = e
+                if x in known:
+                    out.append(f"{pad}a_{x} += ({cond}) ? {ab} : 0.0;")
+                if y in known:
+                    out.append(f"{pad}a_{y} += ({cond}) ? 0.0 : {ab};")
+                continue
+            if kind == "div":
+                n, d = e
+                known = set(self.syms.values())
+                out.append(f"{pad}{{ const double q_ = {ab} / {cc(d)};")
+                if n in known:
+                    out.append(f"{pad}  a_{n} += q_;")
+                if d in known:
+                    out.append(f"{pad}  a_{d} -= q_ * {s}; }}")
+                else:
+                    out.append(f"{pad}}}")
+                continue
+            if kind == "sqrt":
+                if e in set(self.syms.values()):
+                    out.append(f"{pad}a_{e} += (0.5 * {ab}) / {s};")
+                continue
             if kind in ("min", "max"):
                 a, b = e.args
                 op = "<" if kind == "min" else ">"
@@ -222,22 +300,26 @@ def gen_hllc():
     vl = [g.inp(f"vl{k}") for k in range(5)]
     vr = [g.inp(f"vr{k}") for k in range(5)]
     gm = sp.Symbol("gm", real=True)
+    ig1 = sp.Symbol("ig1", real=True)  # 1/(gm-1), computed once in the prologue
     B = g.root
     acc = sp.Float(1.03)
     epsi = 1.0e-10
 
+    # The conserved states and physical fluxes in primitive form: u1/u0 = v1
+    # etc. hold exactly in real arithmetic, so the map (and its derivative)
+    # is the reference's, evaluated with fewer divisions.
     def side(v, t):
         u0 = B.let(f"u{t}0", v[0])
         u1 = B.let(f"u{t}1", v[0] * v[1])
         u2 = B.let(f"u{t}2", v[0] * v[2])
         u3 = B.let(f"u{t}3", v[0] * v[3])
-        u4 = B.let(f"u{t}4", v[4] / (gm - 1) + sp.Rational(1, 2) * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2))
+        u4 = B.let(f"u{t}4", v[4] * ig1 + sp.Rational(1, 2) * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2))
         u = [u0, u1, u2, u3, u4]
         p = v[4]
-        f = [B.let(f"f{t}0", u1), B.let(f"f{t}1", p + u1 ** 2 / u0), B.let(f"f{t}2", u1 * u2 / u0),
-             B.let(f"f{t}3", u1 * u3 / u0), B.let(f"f{t}4", u1 * (u4 + p) / u0)]
+        f = [B.let(f"f{t}0", u1), B.let(f"f{t}1", p + u1 * v[1]), B.let(f"f{t}2", u1 * v[2]),
+             B.let(f"f{t}3", u1 * v[3]), B.let(f"f{t}4", v[1] * (u4 + p))]
         cs = B.let(f"cs{t}", sp.sqrt(gm * p / u0))
-        vn = B.let(f"vn{t}", u1 / u0)
+        vn = v[1]
         return u, f, cs, vn, p
 
     ul, fl, csl, vnl, pl = side(vl, "l")
@@ -252,14 +334,14 @@ def gen_hllc():
     sr = B.let("sr", acc * mx)
     sm = B.let("sm", (ur[1] * (sr - vnr) - ul[1] * (sl - vnl) + pl - pr) / (ur[0] * (sr - vnr) - ul[0] * (sl - vnl)))
     prm = B.let("prm", sm * ul[0] * (sl - vnl) - ul[1] * (sl - vnl) + pl)
-    stl0 = B.let("stl0", ul[0] * (sl - vnl) / (sl - sm))
-    str0 = B.let("str0", ur[0] * (sr - vnr) / (sr - sm))
-    stl = [stl0, B.let("stl1", stl0 * sm), B.let("stl2", stl0 * ul[2] / ul[0]),
-           B.let("stl3", stl0 * ul[3] / ul[0]),
-           B.let("stl4", ((sl - vnl) * ul[4] - pl * vnl + prm * sm) / (sl - sm))]
-    st_r = [str0, B.let("str1", str0 * sm), B.let("str2", str0 * ur[2] / ur[0]),
-            B.let("str3", str0 * ur[3] / ur[0]),
-            B.let("str4", ((sr - vnr) * ur[4] - pr * vnr + prm * sm) / (sr - sm))]
+    il = B.let("il", 1 / (sl - sm))
+    ir = B.let("ir", 1 / (sr - sm))
+    stl0 = B.let("stl0", ul[0] * (sl - vnl) * il)
+    str0 = B.let("str0", ur[0] * (sr - vnr) * ir)
+    stl = [stl0, B.let("stl1", stl0 * sm), B.let("stl2", stl0 * vl[2]), B.let("stl3", stl0 * vl[3]),
+           B.let("stl4", ((sl - vnl) * ul[4] - pl * vnl + prm * sm) * il)]
+    st_r = [str0, B.let("str1", str0 * sm), B.let("str2", str0 * vr[2]), B.let("str3", str0 * vr[3]),
+            B.let("str4", ((sr - vnr) * ur[4] - pr * vnr + prm * sm) * ir)]
     # smooth switches (HLL_rot.f90:679-681): derivative inside the epsi band
     # is kept; outside it they are constants 0/1
     thl = B.let("thl", (sp.Abs(sl) + sl) / sp.Max(2 * sl, epsi))
@@ -269,7 +351,8 @@ def gen_hllc():
         B.let(f"F{k}", thr * fr[k] + thl * fl[k] + (1 - thl - thr) *
               (thm * (fl[k] + sl * (stl[k] - ul[k])) + (1 - thm) * (fr[k] + sr * (st_r[k] - ur[k]))))
     sig = "PIP_HD void hllc_flux_vjp(const double* vl, const double* vr, double gm, const double* lam, double* lvl, double* lvr)"
-    pro = "".join(f"  const double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(5))
+    pro = "".join(f"  const double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(5)) + \
+        "  const double ig1 = 1.0 / (gm - 1.0);\n"
     epi = "".join(f"  lvl[{k}] += a_vl{k};\n  lvr[{k}] += a_vr{k};\n" for k in range(5))
     seeds = [(f"F{k}", f"lam[{k}]") for k in range(5)]
     return g.emit(sig, pro, epi, seeds)
@@ -283,6 +366,7 @@ def gen_hlld():
     vr = [g.inp(f"vr{k}") for k in range(NV)]
     gm = sp.Symbol("gm", real=True)
     ch = sp.Symbol("ch", real=True)
+    ig1 = sp.Symbol("ig1", real=True)  # 1/(gm-1), computed once in the prologue
     h = sp.Float(1.001)
     R = g.root
     half = sp.Rational(1, 2)
@@ -290,7 +374,7 @@ def gen_hlld():
     def side(v, t):
         b2 = R.let(f"b2{t}", v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
         u = [v[0], R.let(f"u{t}1", v[0] * v[1]), R.let(f"u{t}2", v[0] * v[2]), R.let(f"u{t}3", v[0] * v[3]),
-             R.let(f"u{t}4", v[4] / (gm - 1) + half * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) + half * b2),
+             R.let(f"u{t}4", v[4] * ig1 + half * v[0] * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) + half * b2),
              v[5], v[6], v[7], v[8]]
         pt = R.let(f"pt{t}", v[4] + half * b2)
         vb = R.let(f"vb{t}", v[1] * v[5] + v[2] * v[6] + v[3] * v[7])
@@ -329,21 +413,27 @@ def gen_hlld():
     glm(bB, list(fr))
     inv = bC.let("inv", 1 / (sr - sl))
     uh = [bC.let(f"uh{k}", inv * (sr * ur[k] - sl * ul[k] - fr[k] + fl[k])) for k in range(8)]
-    sm = bC.let("sm", uh[1] / uh[0])
+    # shared reciprocals: the star states divide by (s - sm) and by uh0
+    iu0 = bC.let("iu0", 1 / uh[0])
+    sm = bC.let("sm", uh[1] * iu0)
     pt = bC.let("pt", ptl + vl[0] * (sl - vl[1]) * (sm - vl[1]))
-    rol1 = bC.let("rol1", vl[0] * (sl - vl[1]) / (sl - sm))
-    ror1 = bC.let("ror1", vr[0] * (sr - vr[1]) / (sr - sm))
-    sal = bC.let("sal", sp.Abs(uh[5]) / sp.sqrt(rol1))
-    sar = bC.let("sar", sp.Abs(uh[5]) / sp.sqrt(ror1))
+    isl = bC.let("isl", 1 / (sl - sm))
+    isr = bC.let("isr", 1 / (sr - sm))
+    rol1 = bC.let("rol1", vl[0] * (sl - vl[1]) * isl)
+    ror1 = bC.let("ror1", vr[0] * (sr - vr[1]) * isr)
+    rls = bC.let("rls", sp.sqrt(rol1))
+    rrs = bC.let("rrs", sp.sqrt(ror1))
+    sal = bC.let("sal", sp.Abs(uh[5]) / rls)
+    sar = bC.let("sar", sp.Abs(uh[5]) / rrs)
     cG, cD = bC.branches("(sl >= (sm - 1.001 * sal)) || ((sm + 1.001 * sar) >= sr)", None)
     # ---- HLLC-G fallback
-    vy2 = cG.let("gvy2", uh[2] / uh[0])
-    vz2 = cG.let("gvz2", uh[3] / uh[0])
+    vy2 = cG.let("gvy2", uh[2] * iu0)
+    vz2 = cG.let("gvz2", uh[3] * iu0)
     gL, gR = cG.branches("sm >= 0.0", None)
-    for blk, v, u, f, s, ro1, ptk, vb, t in ((gL, vl, ul, fl, sl, rol1, ptl, vbl, "L"),
-                                             (gR, vr, ur, fr, sr, ror1, ptr, vbr, "R")):
+    for blk, v, u, f, s, ro1, ptk, vb, t, isd in ((gL, vl, ul, fl, sl, rol1, ptl, vbl, "L", isl),
+                                                  (gR, vr, ur, fr, sr, ror1, ptr, vbr, "R", isr)):
         u24 = blk.let(f"g{t}u24", ((s - v[1]) * u[4] - ptk * v[1] + pt * sm +
-                                   uh[5] * (vb - sm * uh[5] - vy2 * uh[6] - vz2 * uh[7])) / (s - sm))
+                                   uh[5] * (vb - sm * uh[5] - vy2 * uh[6] - vz2 * uh[7])) * isd)
         u2 = [None, ro1 * sm, ro1 * vy2, ro1 * vz2, u24]
         outs = [s * (ro1 - v[0]) + f[0]] + [s * (u2[k] - u[k]) + f[k] for k in range(1, 5)] + \
                [s * (uh[k] - v[k]) + f[k] for k in range(5, 8)]
@@ -351,7 +441,7 @@ def gen_hlld():
     # ---- HLLD
     bh2 = cD.let("bh2", uh[5] ** 2)
 
-    def star(v, u, s, ro1, ptk, vb, t):
+    def star(v, u, s, ro1, ptk, vb, t, isd):
         f1 = cD.let(f"d{t}f1", 1 / (v[0] * (s - v[1]) * (s - sm) - bh2))
         e = cD.let(f"d{t}e", v[0] * (s - v[1]) ** 2 - bh2)
         u16 = cD.let(f"d{t}u16", v[6] * f1 * e)
@@ -359,12 +449,12 @@ def gen_hlld():
         vy1 = cD.let(f"d{t}vy1", v[2] - f1 * uh[5] * v[6] * (sm - v[1]))
         vz1 = cD.let(f"d{t}vz1", v[3] - f1 * uh[5] * v[7] * (sm - v[1]))
         u14 = cD.let(f"d{t}u14", ((s - v[1]) * u[4] - ptk * v[1] + pt * sm +
-                                  uh[5] * (vb - sm * uh[5] - vy1 * u16 - vz1 * u17)) / (s - sm))
+                                  uh[5] * (vb - sm * uh[5] - vy1 * u16 - vz1 * u17)) * isd)
         u1 = [ro1, ro1 * sm, ro1 * vy1, ro1 * vz1, u14, uh[5], u16, u17]
         return u1, vy1, vz1
 
-    ul1, vyl1, vzl1 = star(vl, ul, sl, rol1, ptl, vbl, "L")
-    ur1, vyr1, vzr1 = star(vr, ur, sr, ror1, ptr, vbr, "R")
+    ul1, vyl1, vzl1 = star(vl, ul, sl, rol1, ptl, vbl, "L", isl)
+    ur1, vyr1, vzr1 = star(vr, ur, sr, ror1, ptr, vbr, "R", isr)
     sl1 = cD.let("sl1", sm - sal)
     sr1 = cD.let("sr1", sm + sar)
     dL, dR, dC = cD.branches("sl1 >= 0.0", "sr1 <= 0.0", None)
@@ -372,27 +462,44 @@ def gen_hlld():
         outs = [s * (u1[0] - v[0]) + f[0]] + [s * (u1[k] - u[k]) + f[k] for k in range(1, 5)] + \
                [s * (u1[k] - v[k]) + f[k] for k in range(5, 8)]
         glm(blk, outs)
-    rls = dC.let("rls", sp.sqrt(rol1))
-    rrs = dC.let("rrs", sp.sqrt(ror1))
     fc = dC.let("fc", 1 / (rls + rrs))
     sg = dC.const("sg", "copysign(1.0, uh5)")
     cvy2 = dC.let("cvy2", fc * (rls * vyl1 + rrs * vyr1 + (ur1[6] - ul1[6]) * sg))
     cvz2 = dC.let("cvz2", fc * (rls * vzl1 + rrs * vzr1 + (ur1[7] - ul1[7]) * sg))
     u26 = dC.let("u26", fc * (rls * ur1[6] + rrs * ul1[6] + rls * rrs * (vyr1 - vyl1) * sg))
     u27 = dC.let("u27", fc * (rls * ur1[7] + rrs * ul1[7] + rls * rrs * (vzr1 - vzl1) * sg))
-    cL, cR = dC.branches("sm >= 0.0", None)
-    for blk, v, u, f, s, s1, u1, ros, vy1, vz1, sgn, t in (
-            (cL, vl, ul, fl, sl, sl1, ul1, rls, vyl1, vzl1, -1, "L"),
-            (cR, vr, ur, fr, sr, sr1, ur1, rrs, vyr1, vzr1, 1, "R")):
-        u24 = blk.let(f"c{t}u24", u1[4] + sgn * ros * (vy1 * u1[6] + vz1 * u1[7] - cvy2 * u26 - cvz2 * u27) * sg)
-        u2 = [u1[0], u1[0] * sm, u1[0] * cvy2, u1[0] * cvz2, u24, uh[5], u26, u27]
-        outs = [s1 * u2[0] - (s1 - s) * u1[0] - s * v[0] + f[0]] + \
-               [s1 * u2[k] - (s1 - s) * u1[k] - s * u[k] + f[k] for k in range(1, 5)] + \
-               [s1 * u2[k] - (s1 - s) * u1[k] - s * v[k] + f[k] for k in range(5, 8)]
-        glm(blk, outs)
+    # L** (sm >= 0) and R** fluxes share one code path over the side
+    # operands selected by the sign of sm: adjacent faces of a smooth flow
+    # take both signs, and separate branches would run both per wavefront.
+    cond = "sm >= 0.0"
+
+    def pick(name, a, b):
+        return dC.select(name, cond, a, b)
+    v = {k: pick(f"cv{k}", vl[k], vr[k]) for k in (0, 5, 6, 7)}
+    u = {k: pick(f"cu{k}", ul[k], ur[k]) for k in range(1, 5)}
+    f = {k: pick(f"cf{k}", fl[k], fr[k]) for k in (0, 1, 2, 3, 4, 6, 7)}
+    f[5] = sp.Integer(0)  # overridden by the GLM term
+    s_ = pick("cs", sl, sr)
+    s1 = pick("cs1", sl1, sr1)
+    ro1 = pick("cro1", rol1, ror1)
+    vy1 = pick("cvy1", vyl1, vyr1)
+    vz1 = pick("cvz1", vzl1, vzr1)
+    u14 = pick("cu14", ul1[4], ur1[4])
+    u16 = pick("cu16", ul1[6], ur1[6])
+    u17 = pick("cu17", ul1[7], ur1[7])
+    ros = pick("cros", rls, rrs)
+    sgn = dC.const("csgn", "((sm >= 0.0) ? -1.0 : 1.0)")
+    u1 = [ro1, ro1 * sm, ro1 * vy1, ro1 * vz1, u14, uh[5], u16, u17]
+    u24 = dC.let("cu24", u1[4] + sgn * ros * (vy1 * u1[6] + vz1 * u1[7] - cvy2 * u26 - cvz2 * u27) * sg)
+    u2 = [u1[0], u1[0] * sm, u1[0] * cvy2, u1[0] * cvz2, u24, uh[5], u26, u27]
+    outs = [s1 * u2[0] - (s1 - s_) * u1[0] - s_ * v[0] + f[0]] + \
+           [s1 * u2[k] - (s1 - s_) * u1[k] - s_ * u[k] + f[k] for k in range(1, 5)] + \
+           [s1 * u2[k] - (s1 - s_) * u1[k] - s_ * v[k] + f[k] for k in range(5, 8)]
+    glm(dC, outs)
     sig = ("PIP_HD void hlld9_flux_vjp(const double* vl, const double* vr, double gm, double ch, "
            "const double* lam, double* lvl, double* lvr)")
-    pro = "".join(f"  double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(9))
+    pro = "".join(f"  double vl{k} = vl[{k}], vr{k} = vr[{k}];\n" for k in range(9)) + \
+        "  const double ig1 = 1.0 / (gm - 1.0);\n"
     epi = "".join(f"  lvl[{k}] += a_vl{k};\n  lvr[{k}] += a_vr{k};\n" for k in range(9))
     seeds = [(f"F{k}", f"lam[{k}]") for k in range(9)]
     return g.emit(sig, pro, epi, seeds, recompute_root=RECOMPUTE)

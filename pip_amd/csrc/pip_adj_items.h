@@ -212,13 +212,51 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   adj_codes(a, dir, FL)[oR] = codes;
 }
 
+// Left / right face-state weights of the cell at position q (0, 1, 2 =
+// v0, v1, v2 of the stencil) for a recorded minmod branch code.
+PIP_HD double wl(unsigned code, int q) {
+  if (q == 0) return (code == 2u) ? -0.5 : 0.0;
+  if (q == 1) return (code == 0u) ? 1.0 : ((code == 1u) ? 0.5 : 1.5);
+  return (code == 1u) ? 0.5 : 0.0;
+}
+PIP_HD double wr(unsigned code, int q) {
+  if (q == 0) return (code == 2u) ? 0.5 : 0.0;
+  if (q == 1) return (code == 0u) ? 1.0 : ((code == 1u) ? 1.5 : 0.5);
+  return (code == 1u) ? -0.5 : 0.0;
+}
+
+// Contributions of the faces of one direction and fluid to the cell:
+// variables n0..n0+NF-1, faces with + cells c+2, c+1, c, c-1 (offsets
+// o2, o1, o0, om, clamped to valid faces) scaled by v2..vm (1 valid, 0 not).
+template <int NF>
+PIP_HD void adj_gather_fluid(const AdjArgs& a, int dir, int fl, int n0, long long o2, long long o1,
+                             long long o0, long long om, double v2, double v1, double v0, double vm,
+                             unsigned mask, double* lv) {
+  constexpr int NVT = ADJ_NVT;
+  const long long N = a.N;
+  const double* PL = a.P + (long long)dir * 2 * NVT * N;
+  const double* PR = PL + (long long)NVT * N;
+  const unsigned long long* C = adj_codes(a, dir, fl);
+  const unsigned long long k2 = C[o2], k1 = C[o1], k0 = C[o0], km = C[om];
+#pragma unroll
+  for (int q = 0; q < NF; q++) {
+    const int n = n0 + q, sh = 4 * q;
+    const double acc = v2 * (PL[n * N + o2] * wl((unsigned)(k2 >> sh) & 3u, 0)) +
+                       v1 * (PL[n * N + o1] * wl((unsigned)(k1 >> sh) & 3u, 1) +
+                             PR[n * N + o1] * wr((unsigned)(k1 >> (sh + 2)) & 3u, 0)) +
+                       v0 * (PL[n * N + o0] * wl((unsigned)(k0 >> sh) & 3u, 2) +
+                             PR[n * N + o0] * wr((unsigned)(k0 >> (sh + 2)) & 3u, 1)) +
+                       vm * (PR[n * N + om] * wr((unsigned)(km >> (sh + 2)) & 3u, 2));
+    lv[n] += ((mask >> n) & 1u) ? -acc : acc;
+  }
+}
+
 // Sum of the face contributions to the primitives of the cell at (i, j),
 // with the sign map `mask` of a ghost image applied.  A face with + cell r
 // (stride st) takes V_L from cells r-2, r-1, r and V_R from r-1, r, r+1,
 // with the weights of the recorded minmod branches.
 PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double* lv) {
-  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
-  const long long N = a.N;
+  constexpr int NV = ADJ_NV;
   const long long o = (long long)j * a.ix + i;
 #pragma unroll
   for (int dir = 0; dir < 2; dir++) {
@@ -227,31 +265,13 @@ PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double*
     const bool row_ok = dir ? (i >= 2 && i <= a.nx + 1) : (j >= 2 && j <= a.ny + 1);
     if (!row_ok) continue;
     const long long st = dir ? a.ix : 1;
-    const double* PL = a.P + (long long)dir * 2 * NVT * N;
-    const double* PR = PL + (long long)NVT * N;
-    const bool f2 = c + 2 <= hi, f1 = c + 1 >= 2 && c + 1 <= hi, f0 = c >= 2 && c <= hi, fm = c - 1 >= 2;
-#pragma unroll
-    for (int fl = 0; fl < 2; fl++) {
-      const unsigned long long* C = adj_codes(a, dir, fl);
-      const unsigned long long k2 = f2 ? C[o + 2 * st] : 0ull, k1 = f1 ? C[o + st] : 0ull,
-                               k0 = f0 ? C[o] : 0ull, km = fm ? C[o - st] : 0ull;
-      const int n0 = fl ? NV : 0, n1 = fl ? NVT : NV;
-      for (int n = n0; n < n1; n++) {
-        const int sh = 4 * (n - n0);
-        double acc = 0.0;
-        if (f2) acc += PL[n * N + o + 2 * st] * recon_w((unsigned)(k2 >> sh) & 3u, false, 0);
-        if (f1) {
-          acc += PL[n * N + o + st] * recon_w((unsigned)(k1 >> sh) & 3u, false, 1);
-          acc += PR[n * N + o + st] * recon_w((unsigned)(k1 >> (sh + 2)) & 3u, true, 0);
-        }
-        if (f0) {
-          acc += PL[n * N + o] * recon_w((unsigned)(k0 >> sh) & 3u, false, 2);
-          acc += PR[n * N + o] * recon_w((unsigned)(k0 >> (sh + 2)) & 3u, true, 1);
-        }
-        if (fm) acc += PR[n * N + o - st] * recon_w((unsigned)(km >> (sh + 2)) & 3u, true, 2);
-        lv[n] += ((mask >> n) & 1u) ? -acc : acc;
-      }
-    }
+    auto clampf = [&](int r) { return r < 2 ? 2 : (r > hi ? hi : r); };
+    const long long o2 = o + (long long)(clampf(c + 2) - c) * st, o1 = o + (long long)(clampf(c + 1) - c) * st,
+                    o0 = o + (long long)(clampf(c) - c) * st, om = o + (long long)(clampf(c - 1) - c) * st;
+    const double v2 = (c + 2 >= 2 && c + 2 <= hi) ? 1.0 : 0.0, v1 = (c + 1 >= 2 && c + 1 <= hi) ? 1.0 : 0.0,
+                 v0 = (c >= 2 && c <= hi) ? 1.0 : 0.0, vm = (c - 1 >= 2 && c - 1 <= hi) ? 1.0 : 0.0;
+    adj_gather_fluid<NV>(a, dir, 0, 0, o2, o1, o0, om, v2, v1, v0, vm, mask, lv);
+    adj_gather_fluid<5>(a, dir, 1, NV, o2, o1, o0, om, v2, v1, v0, vm, mask, lv);
   }
 }
 
