@@ -257,6 +257,19 @@ def main():
     st = s.status()
     assert st == 0, f"stop_sim flags {st}"
 
+    # measured copy bandwidth on this GPU (1 GiB read + 1 GiB written)
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * 8 * (1 << 27) / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+
     ncell = cfg.ncells
     value = ncell * args.steps / wall
     nv = cfg.nvar_m + cfg.nvar_h
@@ -286,7 +299,8 @@ def main():
                      "kernel": "k_stage (stage 2)", "algorithmic_bytes_per_launch": bytes_stage2,
                      "avg_launch_ms": t2 * 1e3,
                      "stage1_avg_launch_ms": t1 * 1e3,
-                     "stage1_achieved_GBs": bytes_stage1 / t1 / 1e9},
+                     "stage1_achieved_GBs": bytes_stage1 / t1 / 1e9,
+                     "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": ach / copy_gbs},
         "hbm_roofline_cell_updates_frac": value * (5 * nv * 8) / (HBM_PEAK_GBS * 1e9),
     }
     if not args.no_extra:
