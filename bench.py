@@ -244,7 +244,7 @@ def main():
     if args.tile == "auto":
         tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
     else:
-        tile = tuple(int(v) for v in args.tile.split("x"))
+        tile = tuple(int(v) for v in args.tile.split("x"))  # TXxTY or TXxTYxREMAP
     s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.enqueue(args.warmup)
     s.sync()
@@ -293,7 +293,8 @@ def main():
         "data": "synthetic (Orszag-Tang initial state, no checkpoint)",
         "config": {"workload": "orszag_tang_2d_two_fluid", "nx": args.n, "ny": args.n,
                    "nvar": nv, "xi_n": 0.9, "alpha_c": 1.0e3, "flag_divb": 1,
-                   "ssp_rk": 2, "cfl": 0.3, "tile": list(tile)},
+                   "ssp_rk": 2, "cfl": 0.3, "tile": list(tile[:2]),
+                   "xcd_remap": int(tile[2]) if len(tile) > 2 else 0},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "kernel": "k_stage (stage 2)", "algorithmic_bytes_per_launch": bytes_stage2,

@@ -100,8 +100,12 @@ int pip_get_rates(pip_ctx *ctx, double *gm_rec, double *gm_ion, double *arb_heat
  * ensemble driver).  U: nvar_m + 5 planes of ix*jx doubles (U_m then U_h). */
 int pip_device_state(pip_ctx *ctx, double **U, double **ac);
 int pip_get_stream(pip_ctx *ctx, void **stream);
-/* Stage-kernel tile shape (autotuner): tx*ty threads per workgroup. */
+/* Stage-kernel tile shape (autotuner): 16x15, 32x7, 32x15, 64x7, 32x8 or
+ * 16x7 cells per workgroup. */
 int pip_set_tile(pip_ctx *ctx, int tx, int ty);
+/* 1: map workgroups to tiles so that each of the 8 XCDs gets a contiguous
+ * range of tiles (neighbouring halos in one L2); 0: row-major (autotuner). */
+int pip_set_xcd_remap(pip_ctx *ctx, int on);
 /* Average device time (ms) of the last timed enqueue of the dominant kernel;
  * measured with hipEvents on the context's stream. */
 int pip_time_steps(pip_ctx *ctx, int nsteps, double *ms_total, double *ms_stage1, double *ms_stage2);

@@ -11,7 +11,8 @@ import os
 from . import build
 
 CACHE = os.path.join(build.LIBDIR, "autotune.json")
-STAGE_TILES = [(16, 15), (32, 7), (32, 15), (32, 8), (16, 7)]
+STAGE_TILES = [(16, 15), (32, 7), (32, 15), (64, 7), (32, 8), (16, 7)]
+XCD_REMAP = [0, 1]  # row-major tile order, or contiguous tile ranges per XCD
 ENS_TILES = [64, 128, 256, 512]
 
 
@@ -30,23 +31,26 @@ def _save(d):
 
 
 def stage_key(cfg) -> str:
-    return f"stage:{cfg.ndim}:{cfg.nx}x{cfg.ny}:nv{cfg.nvar_m}:ir{cfg.flag_ir}:ac{int(cfg.ac_field)}"
+    return (f"stage2:{cfg.ndim}:{cfg.nx}x{cfg.ny}:nv{cfg.nvar_m}:ir{cfg.flag_ir}:ac{int(cfg.ac_field)}"
+            f":col{cfg.flag_col}:imp{cfg.flag_pip_imp}")
 
 
 def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True):
-    """make_solver(tile) -> PIPSolver ready to step.  Returns (best tile, timings)."""
+    """make_solver((tx, ty, xcd_remap)) -> PIPSolver ready to step.  Returns
+    (best (tx, ty, xcd_remap), timings)."""
     cache = _load()
     key = stage_key(cfg)
     if use_cache and key in cache:
         return tuple(cache[key]["tile"]), cache[key]["ms"]
     timings = {}
     for tile in STAGE_TILES:
-        s = make_solver(tile)
-        s.enqueue(3)
-        s.sync()
-        ev = s.time_steps(steps)
-        timings[f"{tile[0]}x{tile[1]}"] = ev["total_ms"] / steps
-        s.close()
+        for remap in XCD_REMAP:
+            s = make_solver((tile[0], tile[1], remap))
+            s.enqueue(3)
+            s.sync()
+            ev = s.time_steps(steps)
+            timings[f"{tile[0]}x{tile[1]}x{remap}"] = ev["total_ms"] / steps
+            s.close()
     best = min(timings, key=timings.get)
     tile = tuple(int(v) for v in best.split("x"))
     cache[key] = {"tile": list(tile), "ms": timings}

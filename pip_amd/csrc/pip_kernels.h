@@ -58,6 +58,7 @@ struct StageArgs {
   int ix, jx, nx, ny;
   long long N;            // plane stride ix*jx
   int istep, t_order, epi;  // epi: 0 none, 1 one cfl round, 2 two rounds + dt reduction
+  int xcd_remap;          // 1: contiguous tile ranges per XCD
   double col;
   double gmin_safety;     // safety*min(dx,dy,dz)
   double dxyz_min;        // min(dx,dy,dz)
@@ -178,7 +179,7 @@ struct TileGeom {
 // face phase in a single pass of the workgroup (e.g. 16x15 on 256 threads).
 // Returns whether the calling thread owns a cell; r / nanflag receive the
 // cell's dt terms when epi >= 1.
-template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
 __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double cmax, int epi, int i0,
                                            int j0, double* smem, RoundOut& r, int& nanflag) {
   using G = TileGeom<NDIM, NV, TX, TY>;
@@ -208,7 +209,7 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
       for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + o];
 #pragma unroll
       for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + o];
-      if (col_type(a.ph) >= 1) {  // set_collisional's get_Te_* rewrite E (PIP_rot.f90:57-58)
+      if (EXT && col_type(a.ph) >= 1) {  // set_collisional's get_Te_* rewrite E (PIP_rot.f90:57-58)
         double pr, te;
         getpr_mhd(um, pr, te, a.ph);
         getpr_hd(uh, pr, te, a.ph);
@@ -271,7 +272,7 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 #pragma unroll
     for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + oc];
     if (a.ac) acc = a.ac[oc];
-    if (col_type(a.ph) >= 1) {  // set_collisional (PIP_rot.f90:44-97), before set_IR
+    if (EXT && col_type(a.ph) >= 1) {  // set_collisional (PIP_rot.f90:44-97), before set_IR
       acc = set_col_cell(um, uh, a.col, a.ph);
       if (a.ac_out) {
         a.ac_out[oc] = acc;
@@ -287,14 +288,16 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
     // set_dt_coll (Solver_rot.f90:83-93): dt/2 per stage for t_order = 2
     const double dtc = (a.t_order == 2) ? dt * 0.5 : dt;
     double dU0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    source_pip(um, uh, acc, gr, gi, arb, a.ph, a.irc.heat_fac, Sh, Sm, ion_pot, dtc, dU0);
+    Phys phs = a.ph;
+    if (!EXT) phs.pip_imp = 0;  // the default kernels carry no implicit branch
+    source_pip(um, uh, acc, gr, gi, arb, phs, a.irc.heat_fac, Sh, Sm, ion_pot, dtc, dU0);
     if (a.ion_pot_out) a.ion_pot_out[oc] = ion_pot;
     const double ax = dt_sub / a.dx;
     const int fL = tj * (TX + 1) + ti;
 #pragma unroll
     for (int n = 0; n < NVT; n++) {
       double u0 = a.U0[n * N + oc];
-      if (a.ph.pip_imp == 1) {  // U_m0 += dt_c dS, U_h0 -= dt_c dS (PIP_rot.f90:1355-1356)
+      if (EXT && a.ph.pip_imp == 1) {  // U_m0 += dt_c dS, U_h0 -= dt_c dS (PIP_rot.f90:1355-1356)
         if (n < 5) u0 = u0 + dU0[n];
         else if (n >= NV && n < NV + 5) u0 = u0 - dU0[n - NV];
         if (a.U0out) a.U0out[n * N + oc] = u0;
@@ -383,7 +386,7 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 // ------------------------------------------------------------ stage kernel
 // One workgroup per tile.  The look-ahead round of the fused step (epi=2)
 // is skipped on the loop's last step (sc->last), as main.f90 ends there.
-template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   extern __shared__ double smem[];
   DevScalars* sc = a.sc;
@@ -392,12 +395,19 @@ __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   const double cmax = sc->cmax;
   const int epi = (a.epi >= 2 && sc->last) ? 1 : a.epi;
   const int ntx = (a.nx + TX - 1) / TX;
-  const int bid = blockIdx.x;
+  int bid = blockIdx.x;
+  if (a.xcd_remap) {
+    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
+    // contiguous range of tiles, so that neighbouring tiles (which share
+    // halo cells) meet in the same L2
+    const int nb = gridDim.x, q = nb / 8, rr = nb % 8, x = bid % 8, k = bid / 8;
+    bid = x * q + (x < rr ? x : rr) + k;
+  }
   const int i0 = (bid % ntx) * TX;
   const int j0 = (bid / ntx) * TY;
   RoundOut r;
   int nanflag;
-  bool valid = stage_tile<NDIM, NV, IR, TX, TY, NT>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
+  bool valid = stage_tile<NDIM, NV, IR, EXT, TX, TY, NT>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
   if (a.epi >= 1) {
     __syncthreads();
     block_reduce_store<NT>(r, nanflag, valid, sc, smem);
@@ -686,6 +696,7 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
     a.ac = nullptr;
     a.ac_out = nullptr;
     a.U0out = nullptr;
+    a.xcd_remap = 0;
     a.arb_heat = nullptr;
     a.ion_pot_out = nullptr;
     a.sc = nullptr;
@@ -734,7 +745,7 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
       a.Uout = U1;
       a.epi = 0;
       for (int i0 = 0; i0 < M.nx; i0 += TC) {
-        stage_tile<1, 8, false, TC, 1, TX>(a, dt, 0.0, 0, i0, 0, smem, r, nanflag);
+        stage_tile<1, 8, false, false, TC, 1, TX>(a, dt, 0.0, 0, i0, 0, smem, r, nanflag);
         __syncthreads();
       }
       __threadfence_block();
@@ -747,7 +758,7 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
       RoundOut acc = {1.0e300, 0.0, 0.0, 0.0};
       int fl = 0;
       for (int i0 = 0; i0 < M.nx; i0 += TC) {
-        bool v = stage_tile<1, 8, false, TC, 1, TX>(a, dt, 0.0, epi, i0, 0, smem, r, nanflag);
+        bool v = stage_tile<1, 8, false, false, TC, 1, TX>(a, dt, 0.0, epi, i0, 0, smem, r, nanflag);
         if (v) {
           acc.dtc = dmin(acc.dtc, r.dtc);
           acc.rhoac = dmax(acc.rhoac, r.rhoac);

@@ -49,6 +49,7 @@ struct pip_ctx {
   DevScalars* sc = nullptr;
   hipStream_t stream = nullptr;
   int tx = 16, ty = 15;
+  int xcd_remap = 0;  // contiguous tile ranges per XCD (autotuned)
   Phys ph;
   IRConst irc;
   Bnd bnd;
@@ -104,6 +105,7 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   a.istep = istep;
   a.t_order = c->cfg.t_order;
   a.epi = epi;
+  a.xcd_remap = c->xcd_remap;
   a.col = c->cfg.col;
   a.gmin_safety = c->gmin_safety;
   a.dxyz_min = c->dxyz_min;
@@ -116,11 +118,11 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   return a;
 }
 
-template <int NDIM, int NV, bool IR, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
 static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   constexpr size_t lds = sizeof(double) * (size_t)TileGeom<NDIM, NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage<NDIM, NV, IR, TX, TY, NT>;
+  auto kern = k_stage<NDIM, NV, IR, EXT, TX, TY, NT>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
@@ -132,31 +134,38 @@ static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   return PIP_OK;
 }
 
-template <int NDIM, int NV, bool IR>
+template <int NDIM, int NV, bool IR, bool EXT>
 static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   if constexpr (NDIM == 1) {
-    return launch_stage_t<1, NV, IR, 255, 1, 256>(c, a);
+    return launch_stage_t<1, NV, IR, EXT, 255, 1, 256>(c, a);
   } else {
     // tile shapes (cells) and workgroup sizes; the single-pass shapes keep
     // every face phase within one pass of the workgroup
-    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, 16, 15, 256>(c, a);
-    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, 32, 7, 256>(c, a);
-    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, 32, 15, 512>(c, a);
-    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, 32, 8, 256>(c, a);
-    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, 16, 7, 128>(c, a);
+    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 16, 15, 256>(c, a);
+    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 32, 7, 256>(c, a);
+    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 32, 15, 512>(c, a);
+    if (c->tx == 64 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 64, 7, 512>(c, a);
+    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, EXT, 32, 8, 256>(c, a);
+    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 16, 7, 128>(c, a);
     return fail(PIP_ERR_ARG, "unsupported tile shape");
   }
+}
+
+template <int NDIM, int NV, bool IR>
+static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
+  const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
+  return ext ? launch_stage_tile<NDIM, NV, IR, true>(c, a) : launch_stage_tile<NDIM, NV, IR, false>(c, a);
 }
 
 static int launch_stage(pip_ctx* c, const StageArgs& a) {
   const bool ir = c->cfg.flag_ir >= 2;
   if (c->cfg.ndim == 1) {
     if (c->nv != 8) return fail(PIP_ERR_UNSUPPORTED, "1-D runs have nvar_m = 8");
-    return ir ? launch_stage_tile<1, 8, true>(c, a) : launch_stage_tile<1, 8, false>(c, a);
+    return ir ? launch_stage_ext<1, 8, true>(c, a) : launch_stage_ext<1, 8, false>(c, a);
   }
   if (c->nv == 9)
-    return ir ? launch_stage_tile<2, 9, true>(c, a) : launch_stage_tile<2, 9, false>(c, a);
-  return ir ? launch_stage_tile<2, 8, true>(c, a) : launch_stage_tile<2, 8, false>(c, a);
+    return ir ? launch_stage_ext<2, 9, true>(c, a) : launch_stage_ext<2, 9, false>(c, a);
+  return ir ? launch_stage_ext<2, 8, true>(c, a) : launch_stage_ext<2, 8, false>(c, a);
 }
 
 static RoundArgs round_args(pip_ctx* c, int do_ir, int init_arb, int do_cfl, int read_rates,
@@ -558,7 +567,7 @@ int pip_get_stream(pip_ctx* c, void** stream) {
 
 int pip_set_tile(pip_ctx* c, int tx, int ty) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
-  const int ok[][2] = {{16, 15}, {32, 7}, {32, 15}, {32, 8}, {16, 7}};
+  const int ok[][2] = {{16, 15}, {32, 7}, {32, 15}, {64, 7}, {32, 8}, {16, 7}};
   for (auto& t : ok)
     if (t[0] == tx && t[1] == ty) {
       c->tx = tx;
@@ -566,6 +575,12 @@ int pip_set_tile(pip_ctx* c, int tx, int ty) {
       return PIP_OK;
     }
   return fail(PIP_ERR_ARG, "unsupported tile shape");
+}
+
+int pip_set_xcd_remap(pip_ctx* c, int on) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  c->xcd_remap = on ? 1 : 0;
+  return PIP_OK;
 }
 
 int pip_time_steps(pip_ctx* c, int nsteps, double* ms_total, double* ms_stage1, double* ms_stage2) {

@@ -123,7 +123,7 @@ def test_unfused_procedures_match_fused_loop():
     assert a.time_host == b.scalars()["time"]
 
 
-@pytest.mark.parametrize("tile", [(16, 15), (32, 7), (32, 15), (32, 8), (16, 7)])
+@pytest.mark.parametrize("tile", [(16, 15), (32, 7), (32, 15), (64, 7), (32, 8), (16, 7), (32, 7, 1), (64, 7, 1)])
 def test_tile_shapes_bitwise_equal(tile):
     say("case orszag_tang n=72 tile", tile)
     cfg, uh, um, ac = problems.orszag_tang(72)
