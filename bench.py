@@ -204,7 +204,8 @@ def main():
     ap.add_argument("--tile", type=str, default="auto", help="TXxTY, or auto (autotuner)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
     ap.add_argument("--pmc-summary", metavar="DIR", help="write the HBM-traffic profile summary from "
-                    "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write) and exit")
+                    "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write; the main "
+                    "runs used --tile) and exit")
     ap.add_argument("--only", choices=["batched", "adjoint", "copy"],
                     help="run one extra leg alone (profiling); copy = 1 GiB device copy that "
                          "calibrates the HBM byte counters")
@@ -212,7 +213,8 @@ def main():
     if args.pmc_summary:
         pm = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
               "-d {d}/{sub} -o run -- python3 /root/repo/bench.py {a}")
-        main_args = "--steps 20 --warmup 5 --no-cpu --no-extra --tile 32x7"
+        tile = "32x7" if args.tile == "auto" else args.tile
+        main_args = f"--steps 20 --warmup 5 --no-cpu --no-extra --tile {tile}"
         cmds = {sub: pm.format(c=c, d=args.pmc_summary, sub=sub, a=a) for sub, c, a in (
             ("fetch", "FETCH_SIZE", main_args), ("write", "WRITE_SIZE", main_args),
             ("copy_fetch", "FETCH_SIZE", "--only copy"), ("copy_write", "WRITE_SIZE", "--only copy"))}
