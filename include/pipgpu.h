@@ -129,16 +129,21 @@ int pip_record_steps(pip_ctx *ctx, int n, double *hist);
 /* n steps with dt/cmax taken from hist (frozen-dt replay of a recorded run,
  * bitwise equal to it from the same state); when Un / U1 are non-NULL the
  * stage-1 input U_n and the stage-2 input U1 of step k are saved at plane
- * offset k*(nvar_m+5) of each array */
-int pip_replay_steps(pip_ctx *ctx, int n, const double *hist, double *Un, double *U1);
+ * offset k*(nvar_m+5) of each array, and when Vn / V1 are non-NULL (with Un
+ * and U1, in the configuration the adjoint covers) so are their primitives
+ * (u2v, HLL_rot.f90:1136-1186) for pip_adj_step.  On return time and the
+ * step count have advanced by the n steps. */
+int pip_replay_steps(pip_ctx *ctx, int n, const double *hist, double *Un, double *U1, double *Vn,
+                     double *V1);
 /* adjoint seed: add the ghost-cell entries of lam onto the interior cells
  * they image (boundary sign maps), then zero the ghosts */
 int pip_adj_seed(pip_ctx *ctx, double *lam);
 /* one step backwards: on entry lam = dJ/dU_{n+1} (interior cells), on exit
  * lam = dJ/dU_n; lam_ac (may be NULL) accumulates dJ/dalpha_c.  Un, U1 are
- * the saved stage inputs of step n and dtc = &hist[2n]. */
-int pip_adj_step(pip_ctx *ctx, const double *Un, const double *U1, const double *dtc, double *lam,
-                 double *lam_ac);
+ * the saved stage inputs of step n, Vn, V1 their saved primitives (both
+ * NULL: computed here) and dtc = &hist[2n]. */
+int pip_adj_step(pip_ctx *ctx, const double *Un, const double *U1, const double *Vn, const double *V1,
+                 const double *dtc, double *lam, double *lam_ac);
 
 /* ---- batched ensemble (no counterpart in the reference: PIP runs one
  * domain per process).  Independent 1-D two-fluid runs of different sizes,

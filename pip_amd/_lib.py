@@ -92,9 +92,9 @@ _SIGS = {
     "pip_get_state_device": [_V, _V],
     "pip_set_ac_device": [_V, _V],
     "pip_record_steps": [_V, _I, _V],
-    "pip_replay_steps": [_V, _I, _V, _V, _V],
+    "pip_replay_steps": [_V, _I, _V, _V, _V, _V, _V],
     "pip_adj_seed": [_V, _V],
-    "pip_adj_step": [_V, _V, _V, _V, _V, _V],
+    "pip_adj_step": [_V, _V, _V, _V, _V, _V, _V, _V],
     "pip_ens_create": [ctypes.POINTER(PipCfg), ctypes.POINTER(PipEnsMember), _I, _PP],
     "pip_ens_upload": [_V, _I, _P, _P],
     "pip_ens_download": [_V, _I, _P, _P],

@@ -40,7 +40,7 @@ def _ptr(t: torch.Tensor) -> int:
 
 class TwoFluidRun:
     def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
-                 device: str = "cuda:0"):
+                 device: str = "cuda:0", save_primitives: bool = True):
         if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
             raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
         if not cfg.ac_field:
@@ -59,6 +59,11 @@ class TwoFluidRun:
         self.hist = torch.zeros(2 * max(self.nsteps, 1), dtype=torch.float64, device=self.device)
         self._seg_Un = None
         self._seg_U1 = None
+        self._seg_Vn = None
+        self._seg_V1 = None
+        # the replay also saves the stage inputs' primitives (twice the segment
+        # memory, no primitive pass in the adjoint step)
+        self.save_primitives = bool(save_primitives)
 
     # ------------------------------------------------------------ helpers
     def _segments(self):
@@ -122,7 +127,7 @@ class TwoFluidRun:
         """The same run with the recorded dt/cmax sequence held fixed (the map
         whose derivative backward() returns)."""
         self._start(U0, ac)
-        check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(self.hist), None, None))
+        check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(self.hist), None, None, None, None))
         out = self._empty_state()
         check(lib().pip_get_state_device(self.h, _ptr(out)))
         self._lib_done()
@@ -136,18 +141,26 @@ class TwoFluidRun:
         segs = self._segments()
         mmax = max((m for _, m in segs), default=0)
         if mmax and (self._seg_Un is None or self._seg_Un.shape[0] < mmax):
+            # stage inputs and their primitives for one segment (4 states per step)
             self._seg_Un = self._empty_state(mmax)
             self._seg_U1 = self._empty_state(mmax)
+            if self.save_primitives:
+                self._seg_Vn = self._empty_state(mmax)
+                self._seg_V1 = self._empty_state(mmax)
         self._torch_done()
         check(L.pip_set_ac_device(self.h, _ptr(ac)))
         check(L.pip_adj_seed(self.h, _ptr(lam)))
         sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
+        sv = self.save_primitives
         for (k0, m), ck in zip(reversed(segs), reversed(ckpts)):
             check(L.pip_set_state_device(self.h, _ptr(ck)))
             check(L.pip_replay_steps(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
-                                     _ptr(self._seg_U1)))
+                                     _ptr(self._seg_U1), _ptr(self._seg_Vn) if sv else None,
+                                     _ptr(self._seg_V1) if sv else None))
             for k in reversed(range(m)):
                 check(L.pip_adj_step(self.h, _ptr(self._seg_Un) + sb * k, _ptr(self._seg_U1) + sb * k,
+                                     _ptr(self._seg_Vn) + sb * k if sv else None,
+                                     _ptr(self._seg_V1) + sb * k if sv else None,
                                      _ptr(self.hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
         self._lib_done()
         return lam, lam_ac

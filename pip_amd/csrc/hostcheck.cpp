@@ -115,6 +115,7 @@ void hc_adj_step(int ix, int jx, const double* p, const int* flag_bnd, const dou
     a.lam_ac = lam_ac;
     a.ac = ac;
     a.V = V.data();
+    a.make_V = 1;
     a.P = P.data();
     a.dtc = dtc;
     a.ix = ix;

@@ -16,13 +16,15 @@
 // point; the cq2pv / get_Pr rewrites only re-round the state and pass
 // adjoints through unchanged:
 //   mu = lam_out on interior cells (psi component times f)
-//   lam_U0 += c0 mu,  lam_in += c1 mu (s = 2)
+//   lam_U0 = c0 mu,  lam_in = c1 mu (s = 2)
 //   lam_in += dt_sub (dS/dUin)^T mu,  lam_ac += dt_sub (dS/dac)^T mu
-//   per face: lamF = a (mu_R - mu_L) -> H^T -> recon^T: four per-cell
-//             contributions written to their own planes (no atomics, so the
-//             result is deterministic)
-//   per cell: gather those planes at the cell and at its ghost images
-//             (periodic / mirror sign maps), then u2v^T into lam_in.
+//   per face: lamF = a (mu_R - mu_L) -> H^T: the adjoints of the face's
+//             left and right states, written to planes of their own with
+//             the face's minmod branch codes (no atomics, so the result is
+//             deterministic)
+//   per cell: recon^T -- gather those planes from the faces whose stencils
+//             hold the cell and its ghost images (periodic / mirror sign
+//             maps) -- then u2v^T, added to the pointwise part in lam_in.
 #pragma once
 
 #include "pip_adj_gen.h"
@@ -35,12 +37,13 @@ namespace pip {
 struct AdjArgs {
   const double* Uin;      // stage input (nvt planes, ghosts filled)
   const double* lam_out;  // adjoint of the stage output
-  double* lam_in;         // adjoint of Uin (assigned here, completed by the gather)
+  double* lam_in;         // adjoint of Uin (ghosts zeroed by the cell item, interior assigned by the gather)
   double* lam_U0;         // adjoint of U0 (s = 2: assigned c0 mu; aliases lam_in for s = 1)
   const double* lam_base; // s = 1: the stage-2 contribution to lam_U0 (nullable)
   double* lam_ac;         // += adjoint of alpha_c (nullable)
   const double* ac;       // per-cell alpha_c (nullable: constant col)
   double* V;              // nvt planes: primitives of Uin (all cells)
+  int make_V;             // 1: the cell item computes V; 0: V was saved by the replay
   double* P;              // 2 dirs x 2 sides x nvt planes: face-state adjoints,
                           // then 2 dirs x 2 fluids planes of packed routing codes
   const double* dtc;      // device: {dt, cmax} of the forward step
@@ -72,19 +75,20 @@ PIP_HD double adj_mu(const AdjArgs& a, int n, long long o, double psi_fac) {
   return (n == 8) ? m * psi_fac : m;
 }
 
-// Per cell (all cells): primitives of Uin; interior cells also take the
-// SSP blend and the collision source.
+// Per cell (all cells): primitives of Uin for the face items (unless the
+// replay saved them) and, in stage 1, zero ghost entries of lam_in (it
+// receives the step's result).  lam_U0 and the interior of lam_in are
+// assigned by the gather item.
 PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
   if (k >= a.N) return;
   const long long N = a.N;
-  double um[NV], uh[5];
+  if (a.make_V) {
+    double um[NV], uh[5], vm[NV], vh[5];
 #pragma unroll
-  for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + k];
+    for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + k];
 #pragma unroll
-  for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + k];
-  {
-    double vm[NV], vh[5];
+    for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + k];
     u2v_mhd<NV>(um, vm, a.ph);
     u2v_hd(uh, vh, a.ph);
 #pragma unroll
@@ -93,30 +97,35 @@ PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
     for (int n = 0; n < 5; n++) a.V[(NV + n) * N + k] = vh[n];
   }
   const int i = (int)(k % a.ix), j = (int)(k / a.ix);
-  if (!adj_interior(a, i, j)) {
-    // the adjoint lives on interior cells; stage 1 writes the step's result
-    if (a.istep == 1) {
+  if (a.istep == 1 && !adj_interior(a, i, j)) {
 #pragma unroll
-      for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = 0.0;
-    }
-    return;
+    for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = 0.0;
   }
+}
+
+// Pointwise part of lam_in at an interior cell: the SSP blend (s = 1:
+// lam_in == lam_U0, since U_n enters both as U0 and as Uin; s = 2: lam_U0 =
+// c0 mu is written here) plus the collision source VJP, S_m[0..4] = +dS,
+// S_h[0..4] = -dS (PIP_rot.f90:1366-1367); lam_ac accumulates the source's
+// alpha_c part.
+PIP_HD void adj_point_part(const AdjArgs& a, long long k, const double* um, const double* uh, double* t) {
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  const long long N = a.N;
   double c0, c1, dt_sub, psi_fac;
   adj_coeffs(a, c0, c1, dt_sub, psi_fac);
   double mu[NVT];
 #pragma unroll
   for (int n = 0; n < NVT; n++) mu[n] = adj_mu(a, n, k, psi_fac);
-  if (a.istep == 1) {  // lam_in == lam_U0: U_n enters both as U0 and as Uin
+  if (a.istep == 1) {
 #pragma unroll
-    for (int n = 0; n < NVT; n++) a.lam_U0[n * N + k] = (a.lam_base ? a.lam_base[n * N + k] : 0.0) + mu[n];
+    for (int n = 0; n < NVT; n++) t[n] = (a.lam_base ? a.lam_base[n * N + k] : 0.0) + mu[n];
   } else {
 #pragma unroll
     for (int n = 0; n < NVT; n++) {
       a.lam_U0[n * N + k] = c0 * mu[n];
-      a.lam_in[n * N + k] = c1 * mu[n];
+      t[n] = c1 * mu[n];
     }
   }
-  // S_m[0..4] = +dS, S_h[0..4] = -dS (PIP_rot.f90:1366-1367)
   double lam[5];
 #pragma unroll
   for (int n = 0; n < 5; n++) lam[n] = dt_sub * (mu[n] - mu[NV + n]);
@@ -124,9 +133,9 @@ PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
   const double acv = a.ac ? a.ac[k] : a.col;
   source_vjp(um, uh, acv, a.ph.gm, lam, lum, luh, &lac);
 #pragma unroll
-  for (int n = 0; n < 8; n++) a.lam_in[n * N + k] += lum[n];
+  for (int n = 0; n < 8; n++) t[n] += lum[n];
 #pragma unroll
-  for (int n = 0; n < 5; n++) a.lam_in[(NV + n) * N + k] += luh[n];
+  for (int n = 0; n < 5; n++) t[NV + n] += luh[n];
   if (a.lam_ac) a.lam_ac[k] += lac;
 }
 
@@ -186,6 +195,20 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
     const double mL = intL ? adj_mu(a, s, oL, psi_fac) : 0.0;
     lamF[n] = coef * (mR - mL);
   }
+  // the minmod branch codes go out first (nothing of the stencil stays live
+  // across the flux VJP)
+  {
+    unsigned long long codes = 0ull;
+#pragma unroll
+    for (int n = 0; n < NF; n++) {
+      const int s = adj_perm(OFF + n, dir);
+      const double* p = a.V + s * N + oR;
+      const unsigned cl = recon_code(p[-2 * st], p[-st], p[0]);
+      const unsigned cr = recon_code(p[-st], p[0], p[st]);
+      codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
+    }
+    adj_codes(a, dir, FL)[oR] = codes;
+  }
   // density floor of get_Side (HLL_rot.f90:60-61)
   const bool flL = !(vl[0] > a.ph.ro_lim), flR = !(vr[0] > a.ph.ro_lim);
   vl[0] = dmax(vl[0], a.ph.ro_lim);
@@ -198,18 +221,12 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   if (flL) lvl[0] = 0.0;
   if (flR) lvr[0] = 0.0;
   double* P = a.P + (long long)dir * 2 * NVT * N;
-  unsigned long long codes = 0ull;
 #pragma unroll
   for (int n = 0; n < NF; n++) {
     const int s = adj_perm(OFF + n, dir);
     P[s * N + oR] = lvl[n];
     P[(NVT + s) * N + oR] = lvr[n];
-    const double* p = a.V + s * N + oR;
-    const unsigned cl = recon_code(p[-2 * st], p[-st], p[0]);
-    const unsigned cr = recon_code(p[-st], p[0], p[st]);
-    codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
   }
-  adj_codes(a, dir, FL)[oR] = codes;
 }
 
 // Left / right face-state weights of the cell at position q (0, 1, 2 =
@@ -277,7 +294,7 @@ PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double*
 
 // Per interior cell: gather the face contributions at the cell and at its
 // ghost images, then the transpose of u2v (HLL_rot.f90:1136-1186) with the
-// forward point's floors, into lam_in.
+// forward point's floors; lam_in = pointwise part + that.
 PIP_HD void adj_gather_item(const AdjArgs& a, long long k) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
   if (k >= a.N) return;
@@ -290,15 +307,16 @@ PIP_HD void adj_gather_item(const AdjArgs& a, long long k) {
   adj_gather_at(a, ci, cj, 0u, lv);
   for_each_image(ci, cj, a.ix, a.jx, a.bnd,
                  [&](int xi, int yj, unsigned m) { adj_gather_at(a, xi, yj, m, lv); });
-  double u[NVT], lu[NVT];
+  double u[NVT], lu[NVT], t[NVT];
 #pragma unroll
   for (int n = 0; n < NVT; n++) u[n] = a.Uin[n * N + k];
+  adj_point_part(a, k, u, u + NV, t);
 #pragma unroll
   for (int n = 0; n < NVT; n++) lu[n] = 0.0;
   u2v_vjp<NV>(u, lv, a.ph, lu);
   u2v_vjp_hd(u + NV, lv + NV, a.ph, lu + NV);
 #pragma unroll
-  for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] += lu[n];
+  for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = t[n] + lu[n];
 }
 
 // Adjoint seed on the ghost cells of a state: add each ghost's value (with

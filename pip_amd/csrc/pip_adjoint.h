@@ -12,13 +12,47 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
   adj_cell_item(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Stencil kernels cover their index space with 64x4 tiles, one per
+// 256-thread workgroup.  Workgroups are dealt round-robin to the 8 XCDs; the
+// tile order gives each XCD a contiguous range of tiles, so that the rows a
+// tile shares with its neighbours (y stencil) meet in one L2.
+constexpr int ADJ_TW = 64, ADJ_TH = 4;
+
+__device__ __forceinline__ int xcd_order(int bid, int nb) {
+  const int q = nb / 8, rr = nb % 8, x = bid % 8, k = bid / 8;
+  return x * q + (x < rr ? x : rr) + k;
+}
+
+__host__ __device__ inline int adj_tiles(int w, int h) {
+  return ((w + ADJ_TW - 1) / ADJ_TW) * ((h + ADJ_TH - 1) / ADJ_TH);
+}
+
+// tile t of a w x h index space -> this thread's item (x, y); false outside
+__device__ __forceinline__ bool adj_tile_item(int t, int w, int h, int& x, int& y) {
+  const int ntx = (w + ADJ_TW - 1) / ADJ_TW;
+  x = (t % ntx) * ADJ_TW + (int)(threadIdx.x % ADJ_TW);
+  y = (t / ntx) * ADJ_TH + (int)(threadIdx.x / ADJ_TW);
+  return x < w && y < h;
+}
+
+// Faces of both directions of one fluid in one launch: the first
+// adj_tiles(nx+1, ny) tiles are x faces, the rest y faces.
 template <int FL>
-__global__ void __launch_bounds__(256) k_adj_faces(AdjArgs a, int dir) {
-  adj_face_item<FL>(a, dir, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256, FL ? 2 : 1) k_adj_faces(AdjArgs a) {
+  const int ntx = adj_tiles(a.nx + 1, a.ny);
+  int t = xcd_order(blockIdx.x, gridDim.x);
+  const int dir = (t < ntx) ? 0 : 1;
+  if (dir) t -= ntx;
+  const int w = dir ? a.nx : a.nx + 1, h = dir ? a.ny + 1 : a.ny;
+  int x, y;
+  if (!adj_tile_item(t, w, h, x, y)) return;
+  adj_face_item<FL>(a, dir, (long long)y * w + x);
 }
 
 __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
-  adj_gather_item(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  int x, y;
+  if (!adj_tile_item(xcd_order(blockIdx.x, gridDim.x), a.nx, a.ny, x, y)) return;
+  adj_gather_item(a, (long long)(y + 2) * a.ix + (x + 2));
 }
 
 __global__ void __launch_bounds__(256) k_adj_fold_seed(double* lam, int nvt, int ix, int jx, long long N,
@@ -37,6 +71,19 @@ __global__ void k_record_dt(const DevScalars* sc, double* hist, int k) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   hist[2 * k] = sc->active ? sc->dt : 0.0;
   hist[2 * k + 1] = sc->cmax;
+}
+// End of a frozen-dt replay of n steps: the scalars a step-by-step run
+// leaves (time and step count advanced, dt / cmax of the last step).
+__global__ void k_replay_end(DevScalars* sc, const double* hist, int n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t = sc->time;
+  for (int k = 0; k < n; k++) t = t + hist[2 * k];
+  sc->time = t;
+  sc->nt = sc->nt + n;
+  sc->dt = hist[2 * (n - 1)];
+  sc->cmax = hist[2 * (n - 1) + 1];
+  sc->active = 1;
+  sc->last = 0;
 }
 __global__ void k_set_dt(DevScalars* sc, const double* hist, int k) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;

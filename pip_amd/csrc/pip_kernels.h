@@ -54,6 +54,9 @@ struct StageArgs {
   double* U0out;          // implicit collisions: updated stage base for the next stage (nullable)
   const double* arb_heat; // IR type 0
   double* ion_pot_out;    // nullable: ion_pot of this stage
+  double* Vout;           // replay kernels, nullable: primitives of Uin at the tile's cells
+                          // and their ghost images
+  const double* dtc;      // replay kernels: frozen {dt, cmax} of the replayed step
   DevScalars* sc;
   int ix, jx, nx, ny;
   long long N;            // plane stride ix*jx
@@ -179,7 +182,7 @@ struct TileGeom {
 // face phase in a single pass of the workgroup (e.g. 16x15 on 256 threads).
 // Returns whether the calling thread owns a cell; r / nanflag receive the
 // cell's dt terms when epi >= 1.
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY = false>
 __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double cmax, int epi, int i0,
                                            int j0, double* smem, RoundOut& r, int& nanflag) {
   using G = TileGeom<NDIM, NV, TX, TY>;
@@ -226,6 +229,17 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
       for (int n = 0; n < NV; n++) V[(n * HY + hy) * HX + hx] = vm[n];
 #pragma unroll
       for (int n = 0; n < 5; n++) V[((NV + n) * HY + hy) * HX + hx] = vh[n];
+      if (REPLAY && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 && (NDIM == 1 || (hy >= 2 && hy < TY + 2 && gj < a.ny + 2))) {
+        // a cell of this tile: its primitives and their ghost images (u2v
+        // commutes with the boundary sign maps: v = m / rho, the pressure
+        // takes squares)
+        double vv[NVT];
+#pragma unroll
+        for (int n = 0; n < NV; n++) vv[n] = vm[n];
+#pragma unroll
+        for (int n = 0; n < 5; n++) vv[NV + n] = vh[n];
+        store_cell<NDIM, NVT>(a.Vout, N, ix, a.jx, gi, gj, vv, a.bnd);
+      }
     }
   }
   __syncthreads();
@@ -386,14 +400,16 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 // ------------------------------------------------------------ stage kernel
 // One workgroup per tile.  The look-ahead round of the fused step (epi=2)
 // is skipped on the loop's last step (sc->last), as main.f90 ends there.
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY>
 __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   extern __shared__ double smem[];
   DevScalars* sc = a.sc;
-  if (!sc->active) return;
-  const double dt = sc->dt;
-  const double cmax = sc->cmax;
-  const int epi = (a.epi >= 2 && sc->last) ? 1 : a.epi;
+  // REPLAY: a step of a frozen-dt replay ({dt, cmax} from a.dtc, always
+  // active, never the loop's last step, no dt reduction)
+  if (!REPLAY && !sc->active) return;
+  const double dt = REPLAY ? a.dtc[0] : sc->dt;
+  const double cmax = REPLAY ? a.dtc[1] : sc->cmax;
+  const int epi = (!REPLAY && a.epi >= 2 && sc->last) ? 1 : a.epi;
   const int ntx = (a.nx + TX - 1) / TX;
   int bid = blockIdx.x;
   if (a.xcd_remap) {
@@ -407,8 +423,8 @@ __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   const int j0 = (bid / ntx) * TY;
   RoundOut r;
   int nanflag;
-  bool valid = stage_tile<NDIM, NV, IR, EXT, TX, TY, NT>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
-  if (a.epi >= 1) {
+  bool valid = stage_tile<NDIM, NV, IR, EXT, TX, TY, NT, REPLAY>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
+  if (!REPLAY && a.epi >= 1) {
     __syncthreads();
     block_reduce_store<NT>(r, nanflag, valid, sc, smem);
   }
@@ -699,6 +715,8 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
     a.xcd_remap = 0;
     a.arb_heat = nullptr;
     a.ion_pot_out = nullptr;
+    a.Vout = nullptr;
+    a.dtc = nullptr;
     a.sc = nullptr;
     a.ix = M.ix;
     a.jx = 1;

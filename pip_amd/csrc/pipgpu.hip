@@ -96,6 +96,8 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   a.ac_out = (col_type(c->ph) >= 1) ? c->ac : nullptr;
   a.arb_heat = c->arb_heat;
   a.ion_pot_out = ion_out;
+  a.Vout = nullptr;
+  a.dtc = nullptr;
   a.sc = c->sc;
   a.ix = c->cfg.ix;
   a.jx = c->cfg.jx;
@@ -118,11 +120,11 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   return a;
 }
 
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY>
 static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   constexpr size_t lds = sizeof(double) * (size_t)TileGeom<NDIM, NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage<NDIM, NV, IR, EXT, TX, TY, NT>;
+  auto kern = k_stage<NDIM, NV, IR, EXT, TX, TY, NT, REPLAY>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
@@ -134,19 +136,19 @@ static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   return PIP_OK;
 }
 
-template <int NDIM, int NV, bool IR, bool EXT>
+template <int NDIM, int NV, bool IR, bool EXT, bool REPLAY = false>
 static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   if constexpr (NDIM == 1) {
-    return launch_stage_t<1, NV, IR, EXT, 255, 1, 256>(c, a);
+    return launch_stage_t<1, NV, IR, EXT, 255, 1, 256, REPLAY>(c, a);
   } else {
     // tile shapes (cells) and workgroup sizes; the single-pass shapes keep
     // every face phase within one pass of the workgroup
-    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 16, 15, 256>(c, a);
-    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 32, 7, 256>(c, a);
-    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 32, 15, 512>(c, a);
-    if (c->tx == 64 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 64, 7, 512>(c, a);
-    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, EXT, 32, 8, 256>(c, a);
-    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 16, 7, 128>(c, a);
+    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 16, 15, 256, REPLAY>(c, a);
+    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 32, 7, 256, REPLAY>(c, a);
+    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 32, 15, 512, REPLAY>(c, a);
+    if (c->tx == 64 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 64, 7, 512, REPLAY>(c, a);
+    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, EXT, 32, 8, 256, REPLAY>(c, a);
+    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 16, 7, 128, REPLAY>(c, a);
     return fail(PIP_ERR_ARG, "unsupported tile shape");
   }
 }
@@ -154,6 +156,12 @@ static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
 template <int NDIM, int NV, bool IR>
 static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
   const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
+  if (a.dtc) {  // frozen-dt replay kernels: the adjoint configuration only
+    if constexpr (NDIM == 2 && NV == 9 && !IR) {
+      if (!ext) return launch_stage_tile<2, 9, false, false, true>(c, a);
+    }
+    return fail(PIP_ERR_UNSUPPORTED, "replay kernels cover 2-D, flag_divb = 1, flag_IR = 0, explicit collisions");
+  }
   return ext ? launch_stage_tile<NDIM, NV, IR, true>(c, a) : launch_stage_tile<NDIM, NV, IR, false>(c, a);
 }
 
@@ -651,8 +659,8 @@ static int adj_alloc(pip_ctx* c) {
   return PIP_OK;
 }
 
-static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* lam_out, double* lam_in,
-                        double* lam_U0, double* lam_ac, const double* dtc) {
+static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* V, const double* lam_out,
+                        double* lam_in, double* lam_U0, double* lam_ac, const double* dtc) {
   AdjArgs a;
   a.Uin = Uin;
   a.lam_out = lam_out;
@@ -661,7 +669,9 @@ static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* 
   a.lam_base = nullptr;
   a.lam_ac = lam_ac;
   a.ac = c->cfg.ac_field ? c->ac : nullptr;
-  a.V = c->adjV;
+  // primitives saved by the replay (read only), or computed by k_adj_cell
+  a.V = V ? const_cast<double*>(V) : c->adjV;
+  a.make_V = V ? 0 : 1;
   a.P = c->adjP;
   a.dtc = dtc;
   a.ix = c->cfg.ix;
@@ -682,14 +692,11 @@ static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* 
 }
 
 static int adj_stage(pip_ctx* c, const AdjArgs& a) {
-  hipLaunchKernelGGL(k_adj_cell, cell_grid(c), dim3(256), 0, c->stream, a);
-  const long long fx = (long long)(c->nx + 1) * c->ny, fy = (long long)c->nx * (c->ny + 1);
-  const dim3 gx((unsigned)((fx + 255) / 256)), gy((unsigned)((fy + 255) / 256));
-  hipLaunchKernelGGL(k_adj_faces<0>, gx, dim3(256), 0, c->stream, a, 0);
-  hipLaunchKernelGGL(k_adj_faces<1>, gx, dim3(256), 0, c->stream, a, 0);
-  hipLaunchKernelGGL(k_adj_faces<0>, gy, dim3(256), 0, c->stream, a, 1);
-  hipLaunchKernelGGL(k_adj_faces<1>, gy, dim3(256), 0, c->stream, a, 1);
-  hipLaunchKernelGGL(k_adj_gather, cell_grid(c), dim3(256), 0, c->stream, a);
+  if (a.make_V || a.istep == 1) hipLaunchKernelGGL(k_adj_cell, cell_grid(c), dim3(256), 0, c->stream, a);
+  const dim3 gf((unsigned)(adj_tiles(c->nx + 1, c->ny) + adj_tiles(c->nx, c->ny + 1)));
+  hipLaunchKernelGGL(k_adj_faces<0>, gf, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_adj_faces<1>, gf, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)adj_tiles(c->nx, c->ny)), dim3(256), 0, c->stream, a);
   HIPCHK(hipGetLastError());
   return PIP_OK;
 }
@@ -730,27 +737,43 @@ int pip_record_steps(pip_ctx* c, int n, double* hist) {
   return PIP_OK;
 }
 
-int pip_replay_steps(pip_ctx* c, int n, const double* hist, double* Un, double* U1) {
+int pip_replay_steps(pip_ctx* c, int n, const double* hist, double* Un, double* U1, double* Vn, double* V1) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
   if (n < 0 || (n > 0 && !hist)) return fail(PIP_ERR_ARG, "n < 0 or null history");
   if (c->cfg.t_order != 2 && U1) return fail(PIP_ERR_UNSUPPORTED, "stage saves cover t_order = 2");
+  if ((Vn || V1) && !(Un && U1)) return fail(PIP_ERR_ARG, "primitive saves need Un and U1");
   const size_t sb = sizeof(double) * (size_t)c->N * c->nvt;
   TRY(set_loop(c, 1.0e300, (long long)1 << 62, 1, -1));
   c->top_pending = false;
-  if (Un && U1 && c->cfg.t_order == 2 && c->cfg.flag_pip_imp == 0 && n > 0) {
-    // stages write straight into the save buffers: step k reads Un[k],
-    // writes U1[k] and then Un[k+1] (the last step writes the state)
+  // direct: stage outputs go straight into the save buffers (step k reads
+  // Un[k], writes U1[k] and then Un[k+1]; the last step writes the state).
+  // frozen: the replay kernels take the step's {dt, cmax} from hist, so no
+  // per-step scalar kernels run (the adjoint configuration).
+  const bool direct = Un && U1 && c->cfg.t_order == 2 && c->cfg.flag_pip_imp == 0;
+  const bool frozen = direct && c->cfg.ndim == 2 && c->nv == 9 && c->cfg.flag_ir == 0 && col_type(c->ph) == 0;
+  if ((Vn || V1) && !frozen)
+    return fail(PIP_ERR_UNSUPPORTED, "primitive saves cover 2-D, flag_divb = 1, flag_IR = 0, flag_col = 1, explicit collisions");
+  if (direct && n > 0) {
     const size_t S = (size_t)c->N * c->nvt;
     HIPCHK(hipMemcpyAsync(Un, c->U, sb, hipMemcpyDeviceToDevice, c->stream));
     for (int k = 0; k < n; k++) {
-      hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
       double* un = Un + (size_t)k * S;
       double* u1 = U1 + (size_t)k * S;
       double* out = (k + 1 < n) ? Un + (size_t)(k + 1) * S : c->U;
-      TRY(launch_stage(c, stage_args(c, 1, un, un, u1, 0, nullptr)));
-      TRY(launch_stage(c, stage_args(c, 2, u1, un, out, 2, nullptr)));
-      TRY(launch_finalize(c, 1, 0));
+      StageArgs a1 = stage_args(c, 1, un, un, u1, 0, nullptr);
+      StageArgs a2 = stage_args(c, 2, u1, un, out, 2, nullptr);
+      if (frozen) {
+        a1.dtc = a2.dtc = hist + 2 * k;
+        a1.Vout = Vn ? Vn + (size_t)k * S : nullptr;
+        a2.Vout = V1 ? V1 + (size_t)k * S : nullptr;
+      } else {
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
+      }
+      TRY(launch_stage(c, a1));
+      TRY(launch_stage(c, a2));
+      if (!frozen) TRY(launch_finalize(c, 1, 0));
     }
+    if (frozen) hipLaunchKernelGGL(k_replay_end, dim3(1), dim3(64), 0, c->stream, c->sc, hist, n);
     HIPCHK(hipGetLastError());
     return PIP_OK;
   }
@@ -774,15 +797,17 @@ int pip_adj_seed(pip_ctx* c, double* lam) {
   return PIP_OK;
 }
 
-int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* dtc, double* lam, double* lam_ac) {
+int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* Vn, const double* V1,
+                 const double* dtc, double* lam, double* lam_ac) {
   if (!c || !Un || !U1 || !dtc || !lam) return fail(PIP_ERR_ARG, "null argument");
+  if (!Vn != !V1) return fail(PIP_ERR_ARG, "give both saved primitive states or neither");
   TRY(adj_check(c));
   TRY(adj_alloc(c));
   // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1):
   // lam -> lamB (adjoint of U1) and lamC (c0 part of the adjoint of U_n)
-  TRY(adj_stage(c, adj_args(c, 2, U1, lam, c->lamB, c->lamC, lam_ac, dtc)));
+  TRY(adj_stage(c, adj_args(c, 2, U1, V1, lam, c->lamB, c->lamC, lam_ac, dtc)));
   // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB (+ lamC) -> lam
-  AdjArgs a1 = adj_args(c, 1, Un, c->lamB, lam, lam, lam_ac, dtc);
+  AdjArgs a1 = adj_args(c, 1, Un, Vn, c->lamB, lam, lam, lam_ac, dtc);
   a1.lam_base = c->lamC;
   TRY(adj_stage(c, a1));
   return PIP_OK;

@@ -23,7 +23,8 @@ def _build():
     src = os.path.join(build.CSRC, "hostcheck.cpp")
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
             os.path.getmtime(os.path.join(build.CSRC, f))
-            for f in ("hostcheck.cpp", "pip_phys.h", "pip_adj_gen.h", "pip_adj_point.h", "pip_bnd.h")):
+            for f in ("hostcheck.cpp", "pip_phys.h", "pip_adj_gen.h", "pip_adj_point.h", "pip_bnd.h",
+                      "pip_adj_items.h")):
         os.makedirs(build.LIBDIR, exist_ok=True)
         subprocess.check_call(["g++", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-std=c++17",
                                "-o", LIB, src])

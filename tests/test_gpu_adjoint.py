@@ -111,6 +111,40 @@ def test_record_matches_fused_loop():
     run.close()
 
 
+def test_saved_primitives_match_recomputed():
+    """The replay's saved primitives give bitwise the gradient of the
+    adjoint step that recomputes them.  The saving replay (frozen dt, stage
+    outputs written into the save buffers) ends bitwise on the recorded
+    state and advances time and step count as the recorded steps do."""
+    from pip_amd._lib import lib
+    from pip_amd.adjoint import _ptr
+    print("case saved vs recomputed primitives n=48 steps=12", flush=True)
+    grads = []
+    for save in (True, False):
+        cfg, run, base, v0, ac, _ = _setup(48, 12, 5)
+        run.save_primitives = save
+        U0 = build_U0(cfg, base, v0)
+        out, ckpts = run.forward_record(U0, ac)
+        if save:
+            bufs = [run._empty_state(12) for _ in range(4)]
+            run._start(U0, ac)
+            s0 = run.solver.scalars()
+            assert lib().pip_replay_steps(run.h, 12, _ptr(run.hist), *[_ptr(b) for b in bufs]) == 0
+            s1 = run.solver.scalars()
+            rep = run._empty_state()
+            assert lib().pip_get_state_device(run.h, _ptr(rep)) == 0
+            run.solver.sync()
+            assert torch.equal(rep, out)
+            t = s0["time"]
+            for dt in run.dt_history():
+                t = t + float(dt)
+            assert s1["time"] == t and s1["nt"] == s0["nt"] + 12
+        g = torch.sin(out * 3.0)
+        grads.append(run.backward(g, ckpts, ac))
+        run.close()
+    assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+
+
 def _fd_check(n, nsteps, ckpt, ncells, tol, seed=13, eps=1.0e-6, eps_ac=None):
     cfg, run, base, v0, ac, arrays = _setup(n, nsteps, ckpt)
     dev = ac.device
