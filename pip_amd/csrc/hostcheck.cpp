@@ -100,7 +100,7 @@ void hc_adj_step(int ix, int jx, const double* p, const int* flag_bnd, const dou
   ph.ro_lim = p[1];
   ph.pr_lim = p[2];
   ph.safety = p[6];
-  std::vector<double> V(NVT * N), P(4 * NVT * N + 4 * N, 0.0), lamB(NVT * N, 0.0), lamC(NVT * N, 0.0);
+  std::vector<double> V(NVT * N), P(4 * NVT * N + 4 * N, 0.0), lamB(NVT * N, 0.0);
   const double dtc[2] = {p[8], p[9]};
   const Bnd bnd = make_bnd(flag_bnd, 2, ADJ_NV, 1);
   for (long long k = 0; k < N; k++) adj_fold_seed_item(lam, NVT, ix, jx, N, bnd, k);
@@ -110,8 +110,7 @@ void hc_adj_step(int ix, int jx, const double* p, const int* flag_bnd, const dou
     a.Uin = (stage == 2) ? U1 : Un;
     a.lam_out = (stage == 2) ? lam : lamB.data();
     a.lam_in = (stage == 2) ? lamB.data() : lam;
-    a.lam_U0 = (stage == 2) ? lamC.data() : lam;
-    a.lam_base = (stage == 2) ? nullptr : lamC.data();
+    a.lam_next = (stage == 2) ? nullptr : lam;
     a.lam_ac = lam_ac;
     a.ac = ac;
     a.V = V.data();

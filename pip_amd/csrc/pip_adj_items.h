@@ -16,7 +16,9 @@
 // point; the cq2pv / get_Pr rewrites only re-round the state and pass
 // adjoints through unchanged:
 //   mu = lam_out on interior cells (psi component times f)
-//   lam_U0 = c0 mu,  lam_in = c1 mu (s = 2)
+//   lam_U0 = c0 mu,  lam_in = c1 mu (s = 2); s = 1: lam_in = lam_U0 + mu
+//   (U_n enters stage 1 as U0 and as Uin; lam_U0 of stage 2 is formed in
+//   stage 1 from the step's output adjoint)
 //   lam_in += dt_sub (dS/dUin)^T mu,  lam_ac += dt_sub (dS/dac)^T mu
 //   per face: lamF = a (mu_R - mu_L) -> H^T: the adjoints of the face's
 //             left and right states, written to planes of their own with
@@ -38,8 +40,8 @@ struct AdjArgs {
   const double* Uin;      // stage input (nvt planes, ghosts filled)
   const double* lam_out;  // adjoint of the stage output
   double* lam_in;         // adjoint of Uin (ghosts zeroed by the cell item, interior assigned by the gather)
-  double* lam_U0;         // adjoint of U0 (s = 2: assigned c0 mu; aliases lam_in for s = 1)
-  const double* lam_base; // s = 1: the stage-2 contribution to lam_U0 (nullable)
+  const double* lam_next; // s = 1: adjoint of the step's output (stage 2's lam_out; may
+                          // alias lam_in: read at a cell before the cell is written)
   double* lam_ac;         // += adjoint of alpha_c (nullable)
   const double* ac;       // per-cell alpha_c (nullable: constant col)
   double* V;              // nvt planes: primitives of Uin (all cells)
@@ -77,8 +79,8 @@ PIP_HD double adj_mu(const AdjArgs& a, int n, long long o, double psi_fac) {
 
 // Per cell (all cells): primitives of Uin for the face items (unless the
 // replay saved them) and, in stage 1, zero ghost entries of lam_in (it
-// receives the step's result).  lam_U0 and the interior of lam_in are
-// assigned by the gather item.
+// receives the step's result).  The interior of lam_in is assigned by the
+// gather item.
 PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
   if (k >= a.N) return;
@@ -103,11 +105,11 @@ PIP_HD void adj_cell_item(const AdjArgs& a, long long k) {
   }
 }
 
-// Pointwise part of lam_in at an interior cell: the SSP blend (s = 1:
-// lam_in == lam_U0, since U_n enters both as U0 and as Uin; s = 2: lam_U0 =
-// c0 mu is written here) plus the collision source VJP, S_m[0..4] = +dS,
-// S_h[0..4] = -dS (PIP_rot.f90:1366-1367); lam_ac accumulates the source's
-// alpha_c part.
+// Pointwise part of lam_in at an interior cell: the SSP blend plus the
+// collision source VJP, S_m[0..4] = +dS, S_h[0..4] = -dS
+// (PIP_rot.f90:1366-1367); lam_ac accumulates the source's alpha_c part.
+// s = 2: c1 mu.  s = 1: (c0 mu') + mu, with mu' the adjoint of the step's
+// output and c0 the stage-2 coefficient (the U0 path of stage 2).
 PIP_HD void adj_point_part(const AdjArgs& a, long long k, const double* um, const double* uh, double* t) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
   const long long N = a.N;
@@ -117,14 +119,17 @@ PIP_HD void adj_point_part(const AdjArgs& a, long long k, const double* um, cons
 #pragma unroll
   for (int n = 0; n < NVT; n++) mu[n] = adj_mu(a, n, k, psi_fac);
   if (a.istep == 1) {
-#pragma unroll
-    for (int n = 0; n < NVT; n++) t[n] = (a.lam_base ? a.lam_base[n * N + k] : 0.0) + mu[n];
-  } else {
+    double d0, d1, dsub2;
+    stage_coeffs(2, a.t_order, a.dtc[0], d0, d1, dsub2);
+    const double pf2 = pexp(-dsub2 / a.dtc[0] * a.safety * a.db_clean);
 #pragma unroll
     for (int n = 0; n < NVT; n++) {
-      a.lam_U0[n * N + k] = c0 * mu[n];
-      t[n] = c1 * mu[n];
+      const double m2 = a.lam_next[n * N + k];
+      t[n] = d0 * ((n == 8) ? m2 * pf2 : m2) + mu[n];
     }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NVT; n++) t[n] = c1 * mu[n];
   }
   double lam[5];
 #pragma unroll

@@ -61,7 +61,6 @@ struct pip_ctx {
   double* adjV = nullptr;   // nvt planes
   double* adjP = nullptr;   // 4 * nvt planes (2 dirs x 2 face sides) + 4 planes of routing codes
   double* lamB = nullptr;   // nvt planes
-  double* lamC = nullptr;   // nvt planes
 };
 
 // --------------------------------------------------------------- helpers
@@ -358,7 +357,6 @@ int pip_destroy(pip_ctx* c) {
   (void)hipFree(c->adjV);
   (void)hipFree(c->adjP);
   (void)hipFree(c->lamB);
-  (void)hipFree(c->lamC);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PIP_OK;
@@ -655,18 +653,16 @@ static int adj_alloc(pip_ctx* c) {
   HIPCHK(hipMalloc(&c->adjV, pb));
   HIPCHK(hipMalloc(&c->adjP, 4 * pb + 4 * sizeof(double) * (size_t)c->N));
   HIPCHK(hipMalloc(&c->lamB, pb));
-  HIPCHK(hipMalloc(&c->lamC, pb));
   return PIP_OK;
 }
 
 static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* V, const double* lam_out,
-                        double* lam_in, double* lam_U0, double* lam_ac, const double* dtc) {
+                        double* lam_in, const double* lam_next, double* lam_ac, const double* dtc) {
   AdjArgs a;
   a.Uin = Uin;
   a.lam_out = lam_out;
   a.lam_in = lam_in;
-  a.lam_U0 = lam_U0;
-  a.lam_base = nullptr;
+  a.lam_next = lam_next;
   a.lam_ac = lam_ac;
   a.ac = c->cfg.ac_field ? c->ac : nullptr;
   // primitives saved by the replay (read only), or computed by k_adj_cell
@@ -804,11 +800,11 @@ int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* V
   TRY(adj_check(c));
   TRY(adj_alloc(c));
   // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1):
-  // lam -> lamB (adjoint of U1) and lamC (c0 part of the adjoint of U_n)
-  TRY(adj_stage(c, adj_args(c, 2, U1, V1, lam, c->lamB, c->lamC, lam_ac, dtc)));
-  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB (+ lamC) -> lam
+  // lam -> lamB (adjoint of U1)
+  TRY(adj_stage(c, adj_args(c, 2, U1, V1, lam, c->lamB, nullptr, lam_ac, dtc)));
+  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB and the c0 part of
+  // lam (the U0 path of stage 2) -> lam
   AdjArgs a1 = adj_args(c, 1, Un, Vn, c->lamB, lam, lam, lam_ac, dtc);
-  a1.lam_base = c->lamC;
   TRY(adj_stage(c, a1));
   return PIP_OK;
 }
