@@ -95,6 +95,14 @@ int pip_status(pip_ctx *ctx, int *flags);
 
 /* Rate fields after pip_set_ir (Gm_rec, Gm_ion, arb_heat), for inspection. */
 int pip_get_rates(pip_ctx *ctx, double *gm_rec, double *gm_ion, double *arb_heat);
+/* The collision coefficient field ac(ix,jx,1) (Globalvar.f90:26) as the
+ * output routines see it: the per-cell field, the alpha of the last stage
+ * for flag_col 2..7, or col everywhere for flag_col = 1. */
+int pip_get_ac(pip_ctx *ctx, double *ac);
+/* Restart (IO_rot.f90:585-589): host -> device copy of arb_heat(ix,jx,1)
+ * read from a snapshot; the next pip_prologue keeps it instead of fixing it
+ * from the state (set_IR does that only for a fresh run, PIP_rot.f90:219). */
+int pip_set_arb_heat(pip_ctx *ctx, const double *arb_heat);
 
 /* Device pointers and stream for zero-copy interop (torch.autograd wrapper,
  * ensemble driver).  U: nvar_m + 5 planes of ix*jx doubles (U_m then U_h). */

@@ -83,6 +83,8 @@ _SIGS = {
     "pip_set_time": [_V, _D, _D],
     "pip_status": [_V, _PI],
     "pip_get_rates": [_V, _P, _P, _P],
+    "pip_get_ac": [_V, _P],
+    "pip_set_arb_heat": [_V, _P],
     "pip_device_state": [_V, _PP, _PP],
     "pip_get_stream": [_V, _PP],
     "pip_set_tile": [_V, _I, _I],

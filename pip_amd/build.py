@@ -2,6 +2,8 @@
 cross-compiles for gfx950).
 
   libpipgpu.so      HIP kernels + C-ABI (include/pipgpu.h)
+  libpipio.so       HDF5 snapshots in the reference's t####.h5 layout
+                    (include/pipio.h), against the HDF5 C library of the image
   pip_driver        Fortran driver through the iso_c_binding module
 """
 from __future__ import annotations
@@ -15,7 +17,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpipgpu.so")
+IOLIB = os.path.join(LIBDIR, "libpipio.so")
 FDIR = os.path.join(HERE, "fortran")
+HDF5_PREFIX = os.environ.get("PIP_HDF5_PREFIX", "/opt/conda")
 FBIN = os.path.join(LIBDIR, "pip_driver")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -65,8 +69,23 @@ def build_fortran() -> str | None:
     return FBIN
 
 
+def build_io() -> str | None:
+    """libpipio.so (host C++, HDF5 C API); skipped (returns None) when the
+    HDF5 headers and library are not installed."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = IOLIB + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I" + inc, "-o", tmp,
+          os.path.join(CSRC, "pipio.cpp"), "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir])
+    os.replace(tmp, IOLIB)
+    return IOLIB
+
+
 def build_all(fortran: bool = True):
-    out = {"lib": build_lib()}
+    out = {"lib": build_lib(), "io": build_io()}
     if fortran and os.path.exists(os.path.join(FDIR, "pip_driver.f90")):
         out["fortran"] = build_fortran()
     return out

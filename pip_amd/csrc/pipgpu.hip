@@ -55,6 +55,7 @@ struct pip_ctx {
   Bnd bnd;
   double gmin_safety, dxyz_min;
   bool arb_done = false;
+  bool arb_given = false;  // arb_heat uploaded for a restart: the next prologue keeps it
   bool top_pending = true;
   int device = 0;
   // adjoint work buffers (allocated on the first pip_adj_step)
@@ -418,7 +419,10 @@ int pip_prologue(pip_ctx* c, double* dt) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
   DevScalars hs = host_scalars_init();
   HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
-  c->arb_done = false;
+  // set_IR fixes arb_heat on its first call of a fresh run only
+  // (PIP_rot.f90:219-229: nout = 0 and no restart)
+  c->arb_done = c->arb_given;
+  c->arb_given = false;
   TRY(launch_bnd(c, c->U));        // boundary map of the initial state
   // set_coefficients(U_h,U_m,0) (Initial_rot.f90:140): set_collisional, set_IR
   if (col_type(c->ph) >= 1) TRY(launch_round(c, round_args(c, 0, 0, 0, 0, nullptr, 1)));
@@ -555,6 +559,28 @@ int pip_get_rates(pip_ctx* c, double* gm_rec, double* gm_ion, double* arb_heat) 
   if (gm_ion) HIPCHK(hipMemcpyAsync(gm_ion, c->gm_ion, pb, hipMemcpyDeviceToHost, c->stream));
   if (arb_heat) HIPCHK(hipMemcpyAsync(arb_heat, c->arb_heat, pb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return PIP_OK;
+}
+
+int pip_set_arb_heat(pip_ctx* c, const double* arb_heat) {
+  if (!c || !arb_heat) return fail(PIP_ERR_ARG, "null argument");
+  if (!c->arb_heat || c->cfg.flag_ir_type != 0)
+    return fail(PIP_ERR_STATE, "no arb_heat field (flag_IR = 0 or flag_IR_type != 0)");
+  HIPCHK(hipMemcpyAsync(c->arb_heat, arb_heat, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->arb_given = true;
+  c->arb_done = true;
+  return PIP_OK;
+}
+
+int pip_get_ac(pip_ctx* c, double* ac) {
+  if (!c || !ac) return fail(PIP_ERR_ARG, "null argument");
+  if (c->ac) {
+    HIPCHK(hipMemcpyAsync(ac, c->ac, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {  // flag_col = 1 without a per-cell field: ac = col everywhere (Solver_rot.f90:105)
+    for (long long k = 0; k < c->N; k++) ac[k] = c->cfg.col;
+  }
   return PIP_OK;
 }
 

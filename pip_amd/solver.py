@@ -84,6 +84,26 @@ class PIPSolver:
         check(lib().pip_status(self.h, ctypes.byref(f)))
         return f.value
 
+    def ac_field(self):
+        """The collision coefficient field as the output routines see it."""
+        c = self.cfg
+        out = np.empty((c.jx, c.ix))
+        check(lib().pip_get_ac(self.h, _p(out)))
+        return out
+
+    def set_arb_heat(self, arb_heat):
+        """Upload arb_heat for a restart (kept by the next prologue)."""
+        c = self.cfg
+        a = np.ascontiguousarray(arb_heat, dtype=np.float64).reshape(c.jx, c.ix)
+        check(lib().pip_set_arb_heat(self.h, _p(a)))
+
+    def set_time(self, time: float, dt: float | None = None):
+        """Set the simulation time (restart) and, optionally, dt."""
+        if dt is None:
+            dt = self.scalars()["dt"]
+        check(lib().pip_set_time(self.h, float(time), float(dt)))
+        self.time_host = float(time)
+
     def rates(self):
         c = self.cfg
         out = [np.empty((c.jx, c.ix)) for _ in range(3)]

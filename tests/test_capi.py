@@ -17,6 +17,23 @@ def test_library_exports_every_header_symbol():
     assert b"gfx950" in L.pip_version()
 
 
+def test_snapshot_library_exports_every_header_symbol():
+    import os
+    import re
+
+    from pip_amd import build, snapshot
+    if not os.path.exists(os.path.join(build.HDF5_PREFIX, "include", "hdf5.h")):
+        pytest.skip("no HDF5 C library in this image")
+    if not os.path.exists(build.IOLIB):
+        build.build_io()
+    src = open(os.path.join(os.path.dirname(_lib.HEADER), "pipio.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?\s*(pipio_\w+)\s*\(", src, re.M)))
+    assert names == ["pipio_last_error", "pipio_read", "pipio_write"]
+    L = snapshot.lib()
+    for n in names:
+        assert hasattr(L, n), n
+
+
 @pytest.mark.parametrize("bad", [
     dict(ndim=3), dict(nvar_m=9), dict(t_order=4), dict(flag_ir=1), dict(ix=6),
     dict(flag_bnd=(0, 1, 1, 1)), dict(dx=0.0), dict(flag_col=8), dict(flag_col=0),
