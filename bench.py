@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile", type=str, default="auto", help="TXxTY, or auto (autotuner)")
+    ap.add_argument("--tile", type=str, default="auto", help="TXxTY[xXCD_ORDER[xTWO_PASS_STAGE_MASK]], or auto (autotuner)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
     ap.add_argument("--pmc-summary", metavar="DIR", help="write the HBM-traffic profile summary from "
                     "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write; the main "
@@ -246,7 +246,7 @@ def main():
     if args.tile == "auto":
         tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
     else:
-        tile = tuple(int(v) for v in args.tile.split("x"))  # TXxTY or TXxTYxREMAP
+        tile = tuple(int(v) for v in args.tile.split("x"))  # TXxTY[xREMAP[xTWO_PASS_MASK]]
     s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.enqueue(args.warmup)
     s.sync()
@@ -296,7 +296,8 @@ def main():
         "config": {"workload": "orszag_tang_2d_two_fluid", "nx": args.n, "ny": args.n,
                    "nvar": nv, "xi_n": 0.9, "alpha_c": 1.0e3, "flag_divb": 1,
                    "ssp_rk": 2, "cfl": 0.3, "tile": list(tile[:2]),
-                   "xcd_remap": int(tile[2]) if len(tile) > 2 else 0},
+                   "xcd_remap": int(tile[2]) if len(tile) > 2 else 0,
+                   "two_pass_stages": int(tile[3]) if len(tile) > 3 else 0},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "kernel": "k_stage (stage 2)", "algorithmic_bytes_per_launch": bytes_stage2,

@@ -111,6 +111,12 @@ int pip_get_stream(pip_ctx *ctx, void **stream);
 /* Stage-kernel tile shape (autotuner): 16x15, 32x7, 32x15, 64x7, 32x8 or
  * 16x7 cells per workgroup. */
 int pip_set_tile(pip_ctx *ctx, int tx, int ty);
+/* Bit s-1 of stage_mask set: SSP-RK stage s takes the two fluids one after
+ * the other in a tile of nvar_m planes (three workgroups per CU; tiles 16x15,
+ * 32x7, 16x7; 2-D runs without ionisation/recombination and with explicit
+ * constant or per-cell alpha, other runs keep the one-pass kernel); clear:
+ * both fluids at once.  Bitwise the same results (autotuner). */
+int pip_set_two_pass(pip_ctx *ctx, int stage_mask);
 /* 1: map workgroups to tiles so that each of the 8 XCDs gets a contiguous
  * range of tiles (neighbouring halos in one L2); 0: row-major (autotuner). */
 int pip_set_xcd_remap(pip_ctx *ctx, int on);

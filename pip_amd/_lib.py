@@ -89,6 +89,7 @@ _SIGS = {
     "pip_get_stream": [_V, _PP],
     "pip_set_tile": [_V, _I, _I],
     "pip_set_xcd_remap": [_V, _I],
+    "pip_set_two_pass": [_V, _I],
     "pip_time_steps": [_V, _I, _PD, _PD, _PD],
     "pip_set_state_device": [_V, _V],
     "pip_get_state_device": [_V, _V],

@@ -13,6 +13,7 @@ from . import build
 CACHE = os.path.join(build.LIBDIR, "autotune.json")
 STAGE_TILES = [(16, 15), (32, 7), (32, 15), (64, 7), (32, 8), (16, 7)]
 XCD_REMAP = [0, 1]  # row-major tile order, or contiguous tile ranges per XCD
+TWO_PASS_TILES = [(16, 15), (32, 7), (16, 7)]  # tiles of the fluid-by-fluid stage kernel
 ENS_TILES = [64, 128, 256, 512]
 
 
@@ -32,25 +33,35 @@ def _save(d):
 
 def stage_key(cfg) -> str:
     return (f"stage2:{cfg.ndim}:{cfg.nx}x{cfg.ny}:nv{cfg.nvar_m}:ir{cfg.flag_ir}:ac{int(cfg.ac_field)}"
-            f":col{cfg.flag_col}:imp{cfg.flag_pip_imp}")
+            f":col{cfg.flag_col}:imp{cfg.flag_pip_imp}:twopass")
+
+
+def two_pass_ok(cfg) -> bool:
+    """Configurations the fluid-by-fluid stage kernel covers."""
+    return cfg.ndim == 2 and cfg.flag_ir == 0 and cfg.flag_col == 1 and cfg.flag_pip_imp == 0
 
 
 def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True):
-    """make_solver((tx, ty, xcd_remap)) -> PIPSolver ready to step.  Returns
-    (best (tx, ty, xcd_remap), timings)."""
+    """make_solver((tx, ty, xcd_remap, two_pass)) -> PIPSolver ready to step
+    (two_pass: mask of the SSP-RK stages that take the fluids one at a
+    time).  Returns (best (tx, ty, xcd_remap, two_pass), timings)."""
     cache = _load()
     key = stage_key(cfg)
     if use_cache and key in cache:
         return tuple(cache[key]["tile"]), cache[key]["ms"]
     timings = {}
-    for tile in STAGE_TILES:
-        for remap in XCD_REMAP:
-            s = make_solver((tile[0], tile[1], remap))
-            s.enqueue(3)
-            s.sync()
-            ev = s.time_steps(steps)
-            timings[f"{tile[0]}x{tile[1]}x{remap}"] = ev["total_ms"] / steps
-            s.close()
+    cands = [(t[0], t[1], r, 0) for t in STAGE_TILES for r in XCD_REMAP]
+    if two_pass_ok(cfg):
+        full = (1 << cfg.t_order) - 1
+        cands += [(t[0], t[1], r, m) for t in TWO_PASS_TILES for r in XCD_REMAP
+                  for m in sorted({1 << (cfg.t_order - 1), full})]
+    for cand in cands:
+        s = make_solver(cand)
+        s.enqueue(3)
+        s.sync()
+        ev = s.time_steps(steps)
+        timings["x".join(str(v) for v in cand)] = ev["total_ms"] / steps
+        s.close()
     best = min(timings, key=timings.get)
     tile = tuple(int(v) for v in best.split("x"))
     cache[key] = {"tile": list(tile), "ms": timings}

@@ -430,6 +430,229 @@ __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   }
 }
 
+// ------------------------------------------------ two-pass stage tile
+// The stage of stage_tile for the explicit-collision configuration without
+// ionisation/recombination (2-D), one fluid at a time: the plasma's
+// primitives and fluxes (NV planes), then the neutrals' (5 planes), in one
+// LDS tile of NV planes.  The tile (46 KB at 16x15, against 71 KB for all
+// 14 planes) and the register set of one flux function at a time let three
+// 256-thread workgroups share a CU.  Every value is computed by the same
+// operations as in stage_tile, so the two kernels agree bit for bit.
+template <int NV, int TX, int TY>
+struct TileGeom2 {
+  static constexpr int HX = TX + 4;
+  static constexpr int HY = TY + 4;
+  static constexpr int NFX = (TX + 1) * TY;
+  static constexpr int NFY = TX * (TY + 1);
+  static constexpr int NF = (NFX > NFY) ? NFX : NFY;
+  static constexpr int LDS_DOUBLES = NV * HY * HX + NV * NF;
+};
+
+// One fluid (FL 0: plasma, NV variables from plane 0, HLLD with GLM; FL 1:
+// neutrals, 5 variables from plane NV, HLLC) over the tile: halo ->
+// primitives, x faces, the SSP blend with the x divergence, y faces, the y
+// divergence.  un receives the fluid's part of the updated cell.  The
+// plasma's partial cell value waits out the y faces in the cell's own
+// output entries (read back by the same thread; in place, the stage base
+// there was read first) rather than in registers next to HLLD's.
+template <int FL, int NV, int TX, int TY, int NT, bool REPLAY>
+__device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, double c0, double c1,
+                                           double dt_sub, int i0, int j0, double* smem, bool valid,
+                                           long long oc, int ti, int tj, double* un) {
+  constexpr int NFL = FL ? 5 : NV;  // variables of this fluid
+  constexpr int OFF = FL ? NV : 0;  // its first plane
+  using G = TileGeom2<NV, TX, TY>;
+  constexpr int HX = G::HX, HY = G::HY, NFX = G::NFX, NFY = G::NFY, NF = G::NF;
+  double* V = smem;
+  double* F = smem + NV * HY * HX;
+  const int tid = threadIdx.x;
+  const long long N = a.N;
+  const int ix = a.ix;
+
+  // ---- halo -> primitives (u2v_mhd / u2v_hd act on one fluid each)
+  for (int c = tid; c < HX * HY; c += NT) {
+    int hx = c % HX, hy = c / HX;
+    int gi = i0 + hx, gj = j0 + hy;
+    if (gi < a.ix && gj < a.jx) {
+      long long o = (long long)gj * ix + gi;
+      double u[NFL], v[NFL];
+#pragma unroll
+      for (int n = 0; n < NFL; n++) u[n] = a.Uin[(OFF + n) * N + o];
+      if (FL == 0) u2v_mhd<NV>(u, v, a.ph);
+      else u2v_hd(u, v, a.ph);
+#pragma unroll
+      for (int n = 0; n < NFL; n++) V[(n * HY + hy) * HX + hx] = v[n];
+      if (REPLAY && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 && hy >= 2 && hy < TY + 2 &&
+          gj < a.ny + 2) {
+        // a cell of this tile: its primitives and their ghost images
+#pragma unroll
+        for (int n = 0; n < NFL; n++) a.Vout[(OFF + n) * N + o] = v[n];
+        for_each_image(gi, gj, ix, a.jx, a.bnd, [&](int xi, int yj, unsigned m) {
+          for (int n = 0; n < NFL; n++)
+            a.Vout[(OFF + n) * N + (long long)yj * ix + xi] = ((m >> (OFF + n)) & 1u) ? -v[n] : v[n];
+        });
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- x faces
+  for (int f = tid; f < NFX; f += NT) {
+    int fi = f % (TX + 1), fj = f / (TX + 1);
+    if (i0 + fi > a.nx || j0 + fj >= a.ny) continue;
+    const double* row = V + (fj + 2) * HX;
+    double vl[NFL], vr[NFL];
+#pragma unroll
+    for (int n = 0; n < NFL; n++) {
+      const double* r = row + n * HY * HX + fi;
+      double a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+      vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+      vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+    }
+    vl[0] = dmax(vl[0], a.ph.ro_lim);
+    vr[0] = dmax(vr[0], a.ph.ro_lim);
+    double fl[NFL];
+    if (FL == 0) hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+    else hllc_flux(vl, vr, a.ph.gm, fl);
+#pragma unroll
+    for (int n = 0; n < NFL; n++) F[n * NF + f] = fl[n];
+  }
+  __syncthreads();
+
+  // ---- the SSP blend and the x divergence; the blend takes the stage
+  // input after source_PIP's cq2pv rewrite of this fluid (PIP_rot.f90:1328)
+  if (valid) {
+    double u[NFL], pv[8];
+#pragma unroll
+    for (int n = 0; n < NFL; n++) u[n] = a.Uin[(OFF + n) * N + oc];
+    if (FL == 0) cq2pv_mhd(u, pv, a.ph);
+    else cq2pv_hd(u, pv, a.ph);
+    const double ax = dt_sub / a.dx;
+    const int fL = tj * (TX + 1) + ti;
+#pragma unroll
+    for (int n = 0; n < NFL; n++) {
+      const double u0 = a.U0[(OFF + n) * N + oc];
+      double x;
+      if (a.istep == 1) x = u0 - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+      else x = c0 * u0 + c1 * u[n] - ax * (F[n * NF + fL + 1] - F[n * NF + fL]);
+      if (FL == 0) a.Uout[(OFF + n) * N + oc] = x;
+      else un[n] = x;
+    }
+  }
+  __syncthreads();
+
+  // ---- y faces: permutation (1,3,2,4,5,7,6,8,9) / (1,3,2,4,5)
+  // (HLL_rot.f90:414-434, 315-325) within the fluid
+  for (int f = tid; f < NFY; f += NT) {
+    int fi = f % TX, fj = f / TX;
+    if (i0 + fi >= a.nx || j0 + fj > a.ny) continue;
+    int hx = fi + 2;
+    double vl[NFL], vr[NFL];
+#pragma unroll
+    for (int n = 0; n < NFL; n++) {
+      int src = n;
+      if (n == 1) src = 2; else if (n == 2) src = 1;
+      else if (FL == 0 && n == 5) src = 6; else if (FL == 0 && n == 6) src = 5;
+      const double* col = V + src * HY * HX + hx;
+      double a0 = col[(fj + 0) * HX], a1 = col[(fj + 1) * HX], a2 = col[(fj + 2) * HX],
+             a3 = col[(fj + 3) * HX];
+      vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+      vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+    }
+    vl[0] = dmax(vl[0], a.ph.ro_lim);
+    vr[0] = dmax(vr[0], a.ph.ro_lim);
+    double fl[NFL];
+    if (FL == 0) hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+    else hllc_flux(vl, vr, a.ph.gm, fl);
+#pragma unroll
+    for (int n = 0; n < NFL; n++) {
+      int dst = n;
+      if (n == 1) dst = 2; else if (n == 2) dst = 1;
+      else if (FL == 0 && n == 5) dst = 6; else if (FL == 0 && n == 6) dst = 5;
+      F[dst * NF + f] = fl[n];
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    const double ay = dt_sub / a.dy;
+    const int fB = tj * TX + ti;
+#pragma unroll
+    for (int n = 0; n < NFL; n++) {
+      const double x = (FL == 0) ? a.Uout[(OFF + n) * N + oc] : un[n];
+      un[n] = x - ay * (F[n * NF + fB + TX] - F[n * NF + fB]);
+    }
+  }
+}
+
+template <int NV, int TX, int TY, int NT, bool REPLAY>
+__global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
+  extern __shared__ double smem[];
+  constexpr int NVT = NV + 5;
+  DevScalars* sc = a.sc;
+  if (!REPLAY && !sc->active) return;
+  const double dt = REPLAY ? a.dtc[0] : sc->dt;
+  const double cmax = REPLAY ? a.dtc[1] : sc->cmax;
+  const int epi = (!REPLAY && a.epi >= 2 && sc->last) ? 1 : a.epi;
+  const int ntx = (a.nx + TX - 1) / TX;
+  int bid = blockIdx.x;
+  if (a.xcd_remap) {  // contiguous tile ranges per XCD (as k_stage)
+    const int nb = gridDim.x, q = nb / 8, rr = nb % 8, x = bid % 8, k = bid / 8;
+    bid = x * q + (x < rr ? x : rr) + k;
+  }
+  const int i0 = (bid % ntx) * TX;
+  const int j0 = (bid / ntx) * TY;
+  double c0, c1, dt_sub;
+  stage_coeffs(a.istep, a.t_order, dt, c0, c1, dt_sub);
+  const int tid = threadIdx.x;
+  const int ti = tid % TX, tj = tid / TX;
+  const int ci = i0 + ti + 2, cj = j0 + tj + 2;
+  const bool valid = (tid < TX * TY) && (i0 + ti < a.nx) && (j0 + tj < a.ny);
+  const long long N = a.N;
+  const long long oc = (long long)cj * a.ix + ci;
+  double un[NVT];
+  fluid_pass<0, NV, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un);
+  __syncthreads();  // the neutral pass reuses the tile
+  fluid_pass<1, NV, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un + NV);
+
+  RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
+  int nanflag = 0;
+  if (valid) {
+    // source_PIP on the stage input (it rewrites its copy through cq2pv, as
+    // the passes did for the blend), add_source, Dedner damping
+    double um[NV], uh[5];
+#pragma unroll
+    for (int n = 0; n < NV; n++) um[n] = a.Uin[n * N + oc];
+#pragma unroll
+    for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + oc];
+    const double acc = a.ac ? a.ac[oc] : a.col;
+    double Sm[5], Sh[5], ion_pot;
+    Phys phs = a.ph;
+    phs.pip_imp = 0;
+    source_pip(um, uh, acc, 0.0, 0.0, 0.0, phs, a.irc.heat_fac, Sh, Sm, ion_pot);
+    if (a.ion_pot_out) a.ion_pot_out[oc] = ion_pot;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      un[n] = un[n] + Sm[n] * dt_sub;
+      un[NV + n] = un[NV + n] + Sh[n] * dt_sub;
+    }
+    if (NV == 9) {
+      double fac = pexp(-dt_sub / dt * a.ph.safety * a.db_clean);
+      un[8] = un[8] * fac;
+    }
+    if (epi >= 1) {
+      r = cfl_round<NV, false>(un, un + NV, acc, ion_pot, a);
+      if (epi >= 2) r = cfl_round<NV, false>(un, un + NV, acc, ion_pot, a);
+#pragma unroll
+      for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
+    }
+    store_cell<2, NVT>(a.Uout, N, a.ix, a.jx, ci, cj, un, a.bnd);
+  }
+  if (!REPLAY && a.epi >= 1) {
+    __syncthreads();
+    block_reduce_store<NT>(r, nanflag, valid, sc, smem);
+  }
+}
+
 // ------------------------------------------------------- pointwise kernels
 // Rewrite chain on every cell (ghosts included, as the reference's array
 // statements do): flags select set_IR (+rates/arb_heat) and cfl.

@@ -50,6 +50,7 @@ struct pip_ctx {
   hipStream_t stream = nullptr;
   int tx = 16, ty = 15;
   int xcd_remap = 0;  // contiguous tile ranges per XCD (autotuned)
+  int two_pass = 0;   // bit s-1: stage s takes the fluids one at a time (autotuned)
   Phys ph;
   IRConst irc;
   Bnd bnd;
@@ -153,9 +154,39 @@ static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   }
 }
 
+template <int NV, int TX, int TY, int NT, bool REPLAY>
+static int launch_stage2p_t(pip_ctx* c, const StageArgs& a) {
+  constexpr size_t lds = sizeof(double) * (size_t)TileGeom2<NV, TX, TY>::LDS_DOUBLES;
+  static bool attr_done = false;
+  auto kern = k_stage2p<NV, TX, TY, NT, REPLAY>;
+  if (!attr_done) {
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_done = true;
+  }
+  const int ntx = (c->nx + TX - 1) / TX, nty = (c->ny + TY - 1) / TY;
+  hipLaunchKernelGGL(kern, dim3(ntx * nty), dim3(NT), lds, c->stream, a);
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+// the fluid-by-fluid stage kernel: 2-D, no IR, explicit constant/per-cell alpha
+template <int NV, bool REPLAY>
+static int launch_stage2p(pip_ctx* c, const StageArgs& a) {
+  if (c->tx == 16 && c->ty == 15) return launch_stage2p_t<NV, 16, 15, 256, REPLAY>(c, a);
+  if (c->tx == 32 && c->ty == 7) return launch_stage2p_t<NV, 32, 7, 256, REPLAY>(c, a);
+  if (c->tx == 16 && c->ty == 7) return launch_stage2p_t<NV, 16, 7, 128, REPLAY>(c, a);
+  return fail(PIP_ERR_ARG, "the two-pass stage kernel takes tiles 16x15, 32x7 and 16x7");
+}
+
 template <int NDIM, int NV, bool IR>
 static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
   const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
+  if constexpr (NDIM == 2 && !IR) {
+    if (((c->two_pass >> (a.istep - 1)) & 1) && !ext) {
+      if (!a.dtc) return launch_stage2p<NV, false>(c, a);
+      if constexpr (NV == 9) return launch_stage2p<9, true>(c, a);
+    }
+  }
   if (a.dtc) {  // frozen-dt replay kernels: the adjoint configuration only
     if constexpr (NDIM == 2 && NV == 9 && !IR) {
       if (!ext) return launch_stage_tile<2, 9, false, false, true>(c, a);
@@ -607,6 +638,13 @@ int pip_set_tile(pip_ctx* c, int tx, int ty) {
       return PIP_OK;
     }
   return fail(PIP_ERR_ARG, "unsupported tile shape");
+}
+
+int pip_set_two_pass(pip_ctx* c, int stage_mask) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (stage_mask < 0 || stage_mask > 7) return fail(PIP_ERR_ARG, "stage mask must be in 0..7");
+  c->two_pass = stage_mask;
+  return PIP_OK;
 }
 
 int pip_set_xcd_remap(pip_ctx* c, int on) {

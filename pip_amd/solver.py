@@ -36,7 +36,7 @@ class PIPSolver:
         self.h = h
         self.time_host = 0.0
         if tile is not None:
-            self.set_tile(*tile)  # (tx, ty) or (tx, ty, xcd_remap)
+            self.set_tile(*tile)  # (tx, ty[, xcd_remap[, two_pass stage mask]])
         self.upload(U_h, U_m, ac)
         if prologue:
             self.prologue()
@@ -159,10 +159,14 @@ class PIPSolver:
     def sync(self):
         check(lib().pip_sync(self.h))
 
-    def set_tile(self, tx: int, ty: int, xcd_remap: int | None = None):
+    def set_tile(self, tx: int, ty: int, xcd_remap: int | None = None, two_pass: int | None = None):
+        """Tile shape, XCD tile order and the mask of stages that take the
+        fluids one at a time (bit s-1: stage s)."""
         check(lib().pip_set_tile(self.h, int(tx), int(ty)))
         if xcd_remap is not None:
             check(lib().pip_set_xcd_remap(self.h, int(xcd_remap)))
+        if two_pass is not None:
+            check(lib().pip_set_two_pass(self.h, int(two_pass)))
 
     def time_steps(self, nsteps: int):
         tt, t1, t2 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -13,6 +13,7 @@ import pytest
 
 from oracle.oracle import Oracle
 from pip_amd import problems
+from pip_amd.config import PIPConfig
 from pip_amd.solver import PIPSolver
 from tests.util import interior, rel_l1
 
@@ -135,6 +136,36 @@ def test_tile_shapes_bitwise_equal(tile):
     gh, gm = s.state()
     assert np.array_equal(interior(rm, cfg), interior(gm, cfg))
     assert np.array_equal(interior(rh, cfg), interior(gh, cfg))
+
+
+@pytest.mark.parametrize("tile", [(16, 15, 1, 3), (32, 7, 0, 3), (16, 7, 1, 3), (16, 15, 1, 2), (16, 15, 1, 1)])
+@pytest.mark.parametrize("bnd", [(1, 1, 1, 1), (1, 1, 2, 2), (2, 2, 10, 10)])
+def test_two_pass_stage_bitwise_equal(tile, bnd):
+    """The fluid-by-fluid stage kernel gives the one-pass kernel's result bit
+    for bit (periodic, mirror and outflow sides)."""
+    say("case orszag_tang n=72 two-pass", tile, bnd)
+    cfg, uh, um, ac = problems.orszag_tang(72)
+    cfg = PIPConfig(**{**cfg.__dict__, "flag_bnd": bnd})
+    ref = PIPSolver(cfg, uh, um, ac)
+    ref.advance(nsteps=8)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
+    s.advance(nsteps=8)
+    for a, b in zip(ref.state(), s.state()):
+        assert np.array_equal(a, b)
+    assert ref.scalars() == s.scalars()
+
+
+def test_two_pass_stage_bitwise_at_full_size():
+    say("case orszag_tang n=1024 two-pass vs one-pass, 3 steps")
+    cfg, uh, um, ac = problems.orszag_tang(1024)
+    ref = PIPSolver(cfg, uh, um, ac, tile=(16, 15, 1, 0))
+    ref.advance(nsteps=3)
+    s = PIPSolver(cfg, uh, um, ac, tile=(16, 15, 1, 3))
+    s.advance(nsteps=3)
+    for a, b in zip(ref.state(), s.state()):
+        assert np.array_equal(a, b)
+    ref.close()
+    s.close()
 
 
 def test_full_size_orszag_tang_two_steps_vs_oracle():
