@@ -154,21 +154,23 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     """Gradient of J = |U_T|^2/2 (interior) w.r.t. the initial state and the
     per-cell alpha_c field vs the plain forward run (pip_advance)."""
     import torch
-    from pip_amd import problems
+    from pip_amd import autotune, problems
     from pip_amd.adjoint import TwoFluidRun
     from pip_amd.solver import PIPSolver
     cfg, uh, um, ac = problems.alfven_damping(n)
-    s = PIPSolver(cfg, uh, um, ac)
+    # the forward and the gradient run the same (autotuned) stage kernels
+    tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.advance(5)  # warm-up
     s.close()
-    s = PIPSolver(cfg, uh, um, ac)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.sync()
     t0 = time.perf_counter()
     s.advance(nsteps)
     t_fwd = time.perf_counter() - t0
     s.close()
     dev = torch.device("cuda:0")
-    run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt)
+    run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile)
     U0 = torch.tensor(np.concatenate([um, uh]), device=dev)
     acT = torch.tensor(ac, device=dev)
     mask = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
@@ -189,6 +191,7 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     t_rec, t_bwd = times[-1]
     run.close()
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
+            "tile": list(tile),
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
             "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd}
 
