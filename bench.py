@@ -80,7 +80,9 @@ def write_pmc_summary(d: str, cmds: dict):
     fetch_b = mean(f2) * 1024 * f_scale
     write_b = mean(w2) * 1024 * w_scale
     ncell, nv = 1024 * 1024, 14
-    lines = ["# k_stage (stage 2), Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
+    lines = ["# Stage-2 kernel, Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
+             "(k_stage, or k_stage2p when the tile's two-pass mask has bit 1; each step launches",
+             "stage 1 then stage 2)", "",
              "Commands (each its own pass):", ""]
     lines += [f"    {c}" for c in cmds.values()]
     lines += ["", f"copy calibration: FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
@@ -303,7 +305,8 @@ def main():
                    "two_pass_stages": int(tile[3]) if len(tile) > 3 else 0},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
-                     "kernel": "k_stage (stage 2)", "algorithmic_bytes_per_launch": bytes_stage2,
+                     "kernel": ("k_stage2p" if len(tile) > 3 and (tile[3] >> 1) & 1 else "k_stage") + " (stage 2)",
+                     "algorithmic_bytes_per_launch": bytes_stage2,
                      "avg_launch_ms": t2 * 1e3,
                      "stage1_avg_launch_ms": t1 * 1e3,
                      "stage1_achieved_GBs": bytes_stage1 / t1 / 1e9,
