@@ -33,16 +33,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE_PMC = os.path.join(ROOT, "profiles", "stage2_orszag_tang_1024_pmc.md")
+# committed rocprofv3 --pmc summaries of the stage-2 kernel: the one-pass
+# k_stage and the fluid-by-fluid k_stage2p
+PROFILE_PMC = {False: os.path.join(ROOT, "profiles", "stage2_orszag_tang_1024_pmc.md"),
+               True: os.path.join(ROOT, "profiles", "stage2p_orszag_tang_1024_pmc.md")}
 
 
-def traffic_from_profile():
-    """HBM bytes per stage-2 launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    if not os.path.exists(PROFILE_PMC):
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PROFILE_PMC).read())
-    return float(m.group(1)) if m else None
+def two_pass_stage2(tile) -> bool:
+    return len(tile) > 3 and bool((int(tile[3]) >> 1) & 1)
+
+
+def traffic_from_profile(tile):
+    """(HBM bytes per stage-2 launch, tile it was measured with) from the
+    committed summary of the stage-2 kernel the tile selects (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = PROFILE_PMC[two_pass_stage2(tile)]
+    if not os.path.exists(path):
+        return None, None
+    txt = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    t = re.search(r"measured with tile:\s*(\S+)", txt)
+    return (float(m.group(1)) if m else None), (t.group(1) if t else None)
 
 
 def _pmc_rows(d: str, counter: str):
@@ -59,7 +70,7 @@ def _pmc_rows(d: str, counter: str):
     return [(k, v) for _, k, v in rows]
 
 
-def write_pmc_summary(d: str, cmds: dict):
+def write_pmc_summary(d: str, cmds: dict, tile: str):
     """HBM bytes per stage-2 k_stage launch from four rocprofv3 --pmc runs
     (d/fetch, d/write: the main workload; d/copy_fetch, d/copy_write: the
     1 GiB copy).  FETCH_SIZE / WRITE_SIZE are in KiB per dispatch; each is
@@ -80,9 +91,11 @@ def write_pmc_summary(d: str, cmds: dict):
     fetch_b = mean(f2) * 1024 * f_scale
     write_b = mean(w2) * 1024 * w_scale
     ncell, nv = 1024 * 1024, 14
-    lines = ["# Stage-2 kernel, Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
-             "(k_stage, or k_stage2p when the tile's two-pass mask has bit 1; each step launches",
-             "stage 1 then stage 2)", "",
+    tt = tuple(int(v) for v in tile.split("x"))
+    kern = "k_stage2p (fluid by fluid)" if two_pass_stage2(tt) else "k_stage (one pass)"
+    lines = [f"# Stage-2 kernel {kern}, Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
+             f"measured with tile: {tile}", "",
+             "(each step launches stage 1 then stage 2)", "",
              "Commands (each its own pass):", ""]
     lines += [f"    {c}" for c in cmds.values()]
     lines += ["", f"copy calibration: FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
@@ -92,7 +105,7 @@ def write_pmc_summary(d: str, cmds: dict):
               f"stage-2 HBM bytes per launch (scaled): read {fetch_b:.4e}, write {write_b:.4e}",
               f"algorithmic bytes per launch: read {2 * nv * 8 * ncell:.4e}, write {nv * 8 * ncell:.4e}", "",
               f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}", ""]
-    open(PROFILE_PMC, "w").write("\n".join(lines))
+    open(PROFILE_PMC[two_pass_stage2(tt)], "w").write("\n".join(lines))
     print("\n".join(lines))
 
 
@@ -223,7 +236,7 @@ def main():
         cmds = {sub: pm.format(c=c, d=args.pmc_summary, sub=sub, a=a) for sub, c, a in (
             ("fetch", "FETCH_SIZE", main_args), ("write", "WRITE_SIZE", main_args),
             ("copy_fetch", "FETCH_SIZE", "--only copy"), ("copy_write", "WRITE_SIZE", "--only copy"))}
-        write_pmc_summary(args.pmc_summary, cmds)
+        write_pmc_summary(args.pmc_summary, cmds, tile)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -285,6 +298,7 @@ def main():
     t2 = ev["stage2_ms"] / args.steps * 1e-3
     t1 = ev["stage1_ms"] / args.steps * 1e-3
     ach = bytes_stage2 / t2 / 1e9
+    traffic = traffic_from_profile(tile)
     out = {
         "metric": "FP64 two-fluid cell-updates/s",
         "value": value,
@@ -304,7 +318,7 @@ def main():
                    "xcd_remap": int(tile[2]) if len(tile) > 2 else 0,
                    "two_pass_stages": int(tile[3]) if len(tile) > 3 else 0},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic[0], "traffic_tile": traffic[1],
                      "kernel": ("k_stage2p" if len(tile) > 3 and (tile[3] >> 1) & 1 else "k_stage") + " (stage 2)",
                      "algorithmic_bytes_per_launch": bytes_stage2,
                      "avg_launch_ms": t2 * 1e3,
