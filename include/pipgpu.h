@@ -186,6 +186,12 @@ int pip_ens_run(pip_ens *e, int use_graph, double *ms);
 int pip_ens_result(pip_ens *e, int m, double *time, long *nt, int *status);
 int pip_ens_destroy(pip_ens *e);
 
+/* Diagnostic, no counterpart in the reference: q[k] = n[k] / d[k] and
+ * r[k] = sqrt(n[k]) computed by the device sequences the Riemann solvers,
+ * conversions and cfl terms use (host arrays of count doubles; synchronous;
+ * allocates). */
+int pip_check_arith(const double *n, const double *d, double *q, double *r, long long count);
+
 const char *pip_last_error(void);
 const char *pip_version(void);
 

@@ -106,6 +106,7 @@ _SIGS = {
     "pip_ens_run": [_V, _I, _PD],
     "pip_ens_result": [_V, _I, _PD, _PL, _PI],
     "pip_ens_destroy": [_V],
+    "pip_check_arith": [_P, _P, _P, _P, ctypes.c_longlong],
 }
 
 

@@ -3,7 +3,8 @@
 //
 // Each function restates one reference routine for a single cell or face;
 // the evaluation order of every expression follows the reference so that the
-// device path reproduces the Fortran arithmetic up to FMA contraction.
+// device path reproduces the Fortran arithmetic operation for operation
+// (built without FMA contraction; quotients below).
 // State order: U_m = (rho, mx, my, mz, E, Bx, By, Bz[, psi]),
 //              U_h = (rho, mx, my, mz, E)   (IO_rot.f90:362-374).
 #pragma once
@@ -28,6 +29,48 @@ constexpr double kMuN = 1.0;       // PIP_rot.f90:37
 PIP_HD double dmin(double a, double b) { return (a < b) ? a : b; }
 PIP_HD double dmax(double a, double b) { return (a > b) ? a : b; }
 PIP_HD double sgl(double x) { return (double)(float)x; }  // Fortran default-real literal
+
+// ------------------------------------------------------------ quotients
+// quo(n, d, qrcp(d)) is n / d.  On the device it runs the steps of the
+// general double division sequence -- v_rcp_f64, two Newton-Raphson steps
+// r += r (1 - d r), q = n r, q += r (n - d q), v_div_fixup_f64 -- without
+// its two v_div_scale_f64 range scalings.  Those are the identity unless an
+// operand or the quotient lies at the ends of the exponent range (|n| below
+// 2^-969, d or 1/d subnormal, a subnormal quotient or |n/d| near 2^768);
+// zero, infinite and NaN operands are settled by the fixup alone.  Outside
+// those ends the result is therefore bitwise the IEEE quotient.  Quotients
+// that share a denominator share its refined reciprocal: 4 instructions
+// each instead of 11.  The host build divides.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double qrcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ double quo(double n, double d, double r) {
+  const double q = n * r;
+  return __builtin_amdgcn_div_fixup(__builtin_fma(r, __builtin_fma(-d, q, n), q), d, n);
+}
+// qsqrt(x) is sqrt(x) the same way: the steps of the general double square
+// root sequence -- v_rsq_f64 y, g = x y, h = y/2, r = 1/2 - h g, g += g r,
+// h += h r, two corrections g += h (x - g g), x itself for +-0 and +inf --
+// without its scaling of arguments below 2^-767.
+__device__ __forceinline__ double qsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+  g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+  return __builtin_amdgcn_class(x, 0x260) ? x : g;  // 0x260: -0, +0, +inf
+}
+#else
+PIP_HD double qrcp(double) { return 0.0; }
+PIP_HD double quo(double n, double d, double) { return n / d; }
+PIP_HD double qsqrt(double x) { return sqrt(x); }
+#endif
+PIP_HD double qdiv(double n, double d) { return quo(n, d, qrcp(d)); }
 
 // ------------------------------------------------------ exp / log / pow
 // exp, log and pow as fixed sequences of IEEE double additions and
@@ -138,7 +181,7 @@ PIP_HD double minmod(double u1, double u2, double u3) {
 template <int NV>
 PIP_HD void u2v_mhd(const double* u, double* v, const Phys& p) {
   double v1 = dmax(u[0], p.ro_lim);
-  double roi = 1.0 / v1;
+  double roi = qdiv(1.0, v1);
   v[0] = v1;
   v[1] = u[1] * roi;
   v[2] = u[2] * roi;
@@ -156,7 +199,7 @@ PIP_HD void u2v_mhd(const double* u, double* v, const Phys& p) {
 // u2v_hd, one cell: HLL_rot.f90:1136-1156
 PIP_HD void u2v_hd(const double* u, double* v, const Phys& p) {
   double v1 = dmax(u[0], p.ro_lim);
-  double roi = 1.0 / v1;
+  double roi = qdiv(1.0, v1);
   v[0] = v1;
   v[1] = u[1] * roi;
   v[2] = u[2] * roi;
@@ -170,7 +213,8 @@ PIP_HD void u2v_hd(const double* u, double* v, const Phys& p) {
 // prim = (de, vx, vy, vz, pr, bx, by, bz) after the floors.
 PIP_HD void cq2pv_mhd(double* u, double* prim, const Phys& p) {
   double de = u[0];
-  double vx = u[1] / de, vy = u[2] / de, vz = u[3] / de;
+  const double rde = qrcp(de);
+  double vx = quo(u[1], de, rde), vy = quo(u[2], de, rde), vz = quo(u[3], de, rde);
   double bx = u[5], by = u[6], bz = u[7];
   double pr = (p.gm - 1.0) * ((u[4] - (0.5 * de) * ((vx * vx + vy * vy) + vz * vz)) -
                               0.5 * ((bx * bx + by * by) + bz * bz));
@@ -180,7 +224,7 @@ PIP_HD void cq2pv_mhd(double* u, double* prim, const Phys& p) {
   u[1] = vx * de;
   u[2] = vy * de;
   u[3] = vz * de;
-  u[4] = (pr / (p.gm - 1.0) + (0.5 * de) * ((vx * vx + vy * vy) + vz * vz)) +
+  u[4] = (qdiv(pr, p.gm - 1.0) + (0.5 * de) * ((vx * vx + vy * vy) + vz * vz)) +
          0.5 * ((bx * bx + by * by) + bz * bz);
   prim[0] = de; prim[1] = vx; prim[2] = vy; prim[3] = vz;
   prim[4] = pr; prim[5] = bx; prim[6] = by; prim[7] = bz;
@@ -189,7 +233,8 @@ PIP_HD void cq2pv_mhd(double* u, double* prim, const Phys& p) {
 // cq2pv_hd, one cell, rewrites u: Scheme_rot.f90:132-150
 PIP_HD void cq2pv_hd(double* u, double* prim, const Phys& p) {
   double de = u[0];
-  double vx = u[1] / de, vy = u[2] / de, vz = u[3] / de;
+  const double rde = qrcp(de);
+  double vx = quo(u[1], de, rde), vy = quo(u[2], de, rde), vz = quo(u[3], de, rde);
   double pr = (p.gm - 1.0) * (u[4] - (0.5 * de) * ((vx * vx + vy * vy) + vz * vz));
   pr = dmax(pr, p.pr_lim);
   de = dmax(de, p.ro_lim);
@@ -197,7 +242,7 @@ PIP_HD void cq2pv_hd(double* u, double* prim, const Phys& p) {
   u[1] = vx * de;
   u[2] = vy * de;
   u[3] = vz * de;
-  u[4] = pr / (p.gm - 1.0) + (0.5 * de) * ((vx * vx + vy * vy) + vz * vz);
+  u[4] = qdiv(pr, p.gm - 1.0) + (0.5 * de) * ((vx * vx + vy * vy) + vz * vz);
   prim[0] = de; prim[1] = vx; prim[2] = vy; prim[3] = vz; prim[4] = pr;
 }
 
@@ -221,22 +266,23 @@ PIP_HD void getpr_mhd(double* u, double& pr, double& te, const Phys& p) {
 }
 
 // v2u_1: HLL_rot.f90:1189-1215
-PIP_HD void v2u_hd(const double* v, double* u, double gm) {
+// rg1 = qrcp(gm - 1.0).
+PIP_HD void v2u_hd(const double* v, double* u, double gm, double rg1) {
   u[0] = v[0];
   u[1] = v[0] * v[1];
   u[2] = v[0] * v[2];
   u[3] = v[0] * v[3];
-  u[4] = v[4] / (gm - 1.0) + (0.5 * v[0]) * ((v[1] * v[1] + v[2] * v[2]) + v[3] * v[3]);
+  u[4] = quo(v[4], gm - 1.0, rg1) + (0.5 * v[0]) * ((v[1] * v[1] + v[2] * v[2]) + v[3] * v[3]);
 }
 
 template <int NV>
-PIP_HD void v2u_mhd(const double* v, double* u, double gm) {
+PIP_HD void v2u_mhd(const double* v, double* u, double gm, double rg1) {
   u[0] = v[0];
   u[1] = v[0] * v[1];
   u[2] = v[0] * v[2];
   u[3] = v[0] * v[3];
   u[5] = v[5]; u[6] = v[6]; u[7] = v[7];
-  u[4] = (v[4] / (gm - 1.0) + (0.5 * v[0]) * ((v[1] * v[1] + v[2] * v[2]) + v[3] * v[3])) +
+  u[4] = (quo(v[4], gm - 1.0, rg1) + (0.5 * v[0]) * ((v[1] * v[1] + v[2] * v[2]) + v[3] * v[3])) +
          0.5 * ((v[5] * v[5] + v[6] * v[6]) + v[7] * v[7]);
   if (NV == 9) u[8] = v[8];
 }
@@ -245,41 +291,44 @@ PIP_HD void v2u_mhd(const double* v, double* u, double gm) {
 // hllc_flux: HLL_rot.f90:615-687
 PIP_HD void hllc_flux(const double* v_l, const double* v_r, double gm, double* flx) {
   double u_l[5], u_r[5], flx_l[5], flx_r[5], st_l[5], st_r[5];
-  v2u_hd(v_l, u_l, gm);
-  v2u_hd(v_r, u_r, gm);
+  const double rg1 = qrcp(gm - 1.0);
+  v2u_hd(v_l, u_l, gm, rg1);
+  v2u_hd(v_r, u_r, gm, rg1);
+  const double ra = qrcp(u_l[0]), rb = qrcp(u_r[0]);  // every quotient by rho_L, rho_R
   double pr_l = v_l[4], pr_r = v_r[4];
   flx_l[0] = u_l[1];
   flx_r[0] = u_r[1];
-  flx_l[1] = pr_l + (u_l[1] * u_l[1]) / u_l[0];
-  flx_r[1] = pr_r + (u_r[1] * u_r[1]) / u_r[0];
-  flx_l[2] = u_l[1] * u_l[2] / u_l[0];
-  flx_r[2] = u_r[1] * u_r[2] / u_r[0];
-  flx_l[3] = u_l[1] * u_l[3] / u_l[0];
-  flx_r[3] = u_r[1] * u_r[3] / u_r[0];
-  flx_l[4] = u_l[1] * (u_l[4] + pr_l) / u_l[0];
-  flx_r[4] = u_r[1] * (u_r[4] + pr_r) / u_r[0];
-  double cs_l = sqrt(gm * pr_l / u_l[0]);
-  double cs_r = sqrt(gm * pr_r / u_r[0]);
-  double vn_l = u_l[1] / u_l[0];
-  double vn_r = u_r[1] / u_r[0];
+  flx_l[1] = pr_l + quo(u_l[1] * u_l[1], u_l[0], ra);
+  flx_r[1] = pr_r + quo(u_r[1] * u_r[1], u_r[0], rb);
+  flx_l[2] = quo(u_l[1] * u_l[2], u_l[0], ra);
+  flx_r[2] = quo(u_r[1] * u_r[2], u_r[0], rb);
+  flx_l[3] = quo(u_l[1] * u_l[3], u_l[0], ra);
+  flx_r[3] = quo(u_r[1] * u_r[3], u_r[0], rb);
+  flx_l[4] = quo(u_l[1] * (u_l[4] + pr_l), u_l[0], ra);
+  flx_r[4] = quo(u_r[1] * (u_r[4] + pr_r), u_r[0], rb);
+  double cs_l = qsqrt(quo(gm * pr_l, u_l[0], ra));
+  double cs_r = qsqrt(quo(gm * pr_r, u_r[0], rb));
+  double vn_l = quo(u_l[1], u_l[0], ra);
+  double vn_r = quo(u_r[1], u_r[0], rb);
   double sl = kAcc * dmin(vn_l - cs_l, vn_r - cs_r);
   double sr = kAcc * dmax(vn_l + cs_l, vn_r + cs_r);
-  double sm = (u_r[1] * (sr - vn_r) - u_l[1] * (sl - vn_l) + pr_l - pr_r) /
-              (u_r[0] * (sr - vn_r) - u_l[0] * (sl - vn_l));
+  double sm = qdiv(u_r[1] * (sr - vn_r) - u_l[1] * (sl - vn_l) + pr_l - pr_r,
+                   u_r[0] * (sr - vn_r) - u_l[0] * (sl - vn_l));
   double pr_m = sm * u_l[0] * (sl - vn_l) - u_l[1] * (sl - vn_l) + pr_l;
-  st_l[0] = u_l[0] * (sl - vn_l) / (sl - sm);
-  st_r[0] = u_r[0] * (sr - vn_r) / (sr - sm);
+  const double rsl = qrcp(sl - sm), rsr = qrcp(sr - sm);
+  st_l[0] = quo(u_l[0] * (sl - vn_l), sl - sm, rsl);
+  st_r[0] = quo(u_r[0] * (sr - vn_r), sr - sm, rsr);
   st_l[1] = st_l[0] * sm;
   st_r[1] = st_r[0] * sm;
-  st_l[2] = st_l[0] * u_l[2] / u_l[0];
-  st_r[2] = st_r[0] * u_r[2] / u_r[0];
-  st_l[3] = st_l[0] * u_l[3] / u_l[0];
-  st_r[3] = st_r[0] * u_r[3] / u_r[0];
-  st_l[4] = ((sl - vn_l) * u_l[4] - pr_l * vn_l + pr_m * sm) / (sl - sm);
-  st_r[4] = ((sr - vn_r) * u_r[4] - pr_r * vn_r + pr_m * sm) / (sr - sm);
-  double thl = (fabs(sl) + sl) / dmax(2.0 * sl, kEpsi);
-  double thm = (fabs(sm) + sm) / dmax(2.0 * sm, kEpsi);
-  double thr = 1.0 - (fabs(sr) + sr) / dmax(2.0 * sr, kEpsi);
+  st_l[2] = quo(st_l[0] * u_l[2], u_l[0], ra);
+  st_r[2] = quo(st_r[0] * u_r[2], u_r[0], rb);
+  st_l[3] = quo(st_l[0] * u_l[3], u_l[0], ra);
+  st_r[3] = quo(st_r[0] * u_r[3], u_r[0], rb);
+  st_l[4] = quo((sl - vn_l) * u_l[4] - pr_l * vn_l + pr_m * sm, sl - sm, rsl);
+  st_r[4] = quo((sr - vn_r) * u_r[4] - pr_r * vn_r + pr_m * sm, sr - sm, rsr);
+  double thl = qdiv(fabs(sl) + sl, dmax(2.0 * sl, kEpsi));
+  double thm = qdiv(fabs(sm) + sm, dmax(2.0 * sm, kEpsi));
+  double thr = 1.0 - qdiv(fabs(sr) + sr, dmax(2.0 * sr, kEpsi));
 #pragma unroll
   for (int n = 0; n < 5; n++)
     flx[n] = thr * flx_r[n] + thl * flx_l[n] +
@@ -292,8 +341,9 @@ PIP_HD void hllc_flux(const double* v_l, const double* v_r, double gm, double* f
 template <int NV>
 PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cmax, double* flx) {
   double u_l[NV], u_r[NV], flx_l[8], flx_r[8], u_hll[8];
-  v2u_mhd<NV>(v_l, u_l, gm);
-  v2u_mhd<NV>(v_r, u_r, gm);
+  const double rg1 = qrcp(gm - 1.0);
+  v2u_mhd<NV>(v_l, u_l, gm, rg1);
+  v2u_mhd<NV>(v_r, u_r, gm, rg1);
 
   double b2_l = (v_l[5] * v_l[5] + v_l[6] * v_l[6]) + v_l[7] * v_l[7];
   double pt_l = v_l[4] + 0.5 * b2_l;
@@ -321,10 +371,10 @@ PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cm
 
   double f1 = gm * v_l[4];
   double f2 = 4.0 * f1 * (v_l[5] * v_l[5]);
-  double vf_l = sqrt(((f1 + b2_l) + sqrt(dmax((f1 + b2_l) * (f1 + b2_l) - f2, 0.0))) / (2.0 * v_l[0]));
+  double vf_l = qsqrt(qdiv((f1 + b2_l) + qsqrt(dmax((f1 + b2_l) * (f1 + b2_l) - f2, 0.0)), 2.0 * v_l[0]));
   f1 = gm * v_r[4];
   f2 = 4.0 * f1 * (v_r[5] * v_r[5]);
-  double vf_r = sqrt(((f1 + b2_r) + sqrt(dmax((f1 + b2_r) * (f1 + b2_r) - f2, 0.0))) / (2.0 * v_r[0]));
+  double vf_r = qsqrt(qdiv((f1 + b2_r) + qsqrt(dmax((f1 + b2_r) * (f1 + b2_r) - f2, 0.0)), 2.0 * v_r[0]));
 
   double sl = dmin(v_l[1], v_r[1]) - dmax(vf_l, vf_r);
   double sr = dmax(v_l[1], v_r[1]) + dmax(vf_l, vf_r);
@@ -336,19 +386,21 @@ PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cm
 #pragma unroll
     for (int n = 0; n < 8; n++) flx[n] = flx_r[n];
   } else {
-    f1 = 1.0 / (sr - sl);
+    f1 = qdiv(1.0, sr - sl);
 #pragma unroll
     for (int n = 0; n < 8; n++) u_hll[n] = f1 * (((sr * u_r[n] - sl * u_l[n]) - flx_r[n]) + flx_l[n]);
-    double sm = u_hll[1] / u_hll[0];
+    const double rh = qrcp(u_hll[0]);
+    double sm = quo(u_hll[1], u_hll[0], rh);
     double pt = pt_l + v_l[0] * (sl - v_l[1]) * (sm - v_l[1]);
-    double ro_l1 = v_l[0] * (sl - v_l[1]) / (sl - sm);
-    double ro_r1 = v_r[0] * (sr - v_r[1]) / (sr - sm);
-    double sal = fabs(u_hll[5]) / sqrt(ro_l1);
-    double sar = fabs(u_hll[5]) / sqrt(ro_r1);
+    const double rsl = qrcp(sl - sm), rsr = qrcp(sr - sm);  // ro_l1 / u_l1[4], ro_r1 / u_r1[4]
+    double ro_l1 = quo(v_l[0] * (sl - v_l[1]), sl - sm, rsl);
+    double ro_r1 = quo(v_r[0] * (sr - v_r[1]), sr - sm, rsr);
+    double sal = qdiv(fabs(u_hll[5]), qsqrt(ro_l1));
+    double sar = qdiv(fabs(u_hll[5]), qsqrt(ro_r1));
     if ((sl >= (sm - kHllg * sal)) || ((sm + kHllg * sar) >= sr)) {
       // revert to HLLC-G :775-807
-      double vy_2 = u_hll[2] / u_hll[0];
-      double vz_2 = u_hll[3] / u_hll[0];
+      double vy_2 = quo(u_hll[2], u_hll[0], rh);
+      double vz_2 = quo(u_hll[3], u_hll[0], rh);
       const bool left = (sm >= 0.0);
       const double* v = left ? v_l : v_r;
       const double* uu = left ? u_l : u_r;
@@ -358,9 +410,9 @@ PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cm
       double ptk = left ? pt_l : pt_r;
       double vb = left ? vbL : vbR;
       double u2[5];
-      u2[4] = ((((s - v[1]) * uu[4] - ptk * v[1]) + pt * sm) +
-               u_hll[5] * (((vb - sm * u_hll[5]) - vy_2 * u_hll[6]) - vz_2 * u_hll[7])) /
-              (s - sm);
+      u2[4] = quo((((s - v[1]) * uu[4] - ptk * v[1]) + pt * sm) +
+                      u_hll[5] * (((vb - sm * u_hll[5]) - vy_2 * u_hll[6]) - vz_2 * u_hll[7]),
+                  s - sm, left ? rsl : rsr);
       u2[1] = ro1 * sm;
       u2[2] = ro1 * vy_2;
       u2[3] = ro1 * vz_2;
@@ -373,30 +425,30 @@ PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cm
       // HLLD intermediate states :810-843
       double bh2 = u_hll[5] * u_hll[5];
       double u_l1[8], u_r1[8];
-      f1 = 1.0 / (v_l[0] * (sl - v_l[1]) * (sl - sm) - bh2);
+      f1 = qdiv(1.0, v_l[0] * (sl - v_l[1]) * (sl - sm) - bh2);
       u_l1[5] = u_hll[5];
       u_l1[6] = v_l[6] * f1 * (v_l[0] * ((sl - v_l[1]) * (sl - v_l[1])) - bh2);
       u_l1[7] = v_l[7] * f1 * (v_l[0] * ((sl - v_l[1]) * (sl - v_l[1])) - bh2);
       double vy_l1 = v_l[2] - f1 * u_hll[5] * v_l[6] * (sm - v_l[1]);
       double vz_l1 = v_l[3] - f1 * u_hll[5] * v_l[7] * (sm - v_l[1]);
       u_l1[0] = ro_l1;
-      u_l1[4] = ((((sl - v_l[1]) * u_l[4] - pt_l * v_l[1]) + pt * sm) +
-                 u_hll[5] * (((vbL - sm * u_hll[5]) - vy_l1 * u_l1[6]) - vz_l1 * u_l1[7])) /
-                (sl - sm);
+      u_l1[4] = quo((((sl - v_l[1]) * u_l[4] - pt_l * v_l[1]) + pt * sm) +
+                        u_hll[5] * (((vbL - sm * u_hll[5]) - vy_l1 * u_l1[6]) - vz_l1 * u_l1[7]),
+                    sl - sm, rsl);
       u_l1[1] = ro_l1 * sm;
       u_l1[2] = ro_l1 * vy_l1;
       u_l1[3] = ro_l1 * vz_l1;
 
-      f1 = 1.0 / (v_r[0] * (sr - v_r[1]) * (sr - sm) - bh2);
+      f1 = qdiv(1.0, v_r[0] * (sr - v_r[1]) * (sr - sm) - bh2);
       u_r1[5] = u_hll[5];
       u_r1[6] = v_r[6] * f1 * (v_r[0] * ((sr - v_r[1]) * (sr - v_r[1])) - bh2);
       u_r1[7] = v_r[7] * f1 * (v_r[0] * ((sr - v_r[1]) * (sr - v_r[1])) - bh2);
       double vy_r1 = v_r[2] - f1 * u_hll[5] * v_r[6] * (sm - v_r[1]);
       double vz_r1 = v_r[3] - f1 * u_hll[5] * v_r[7] * (sm - v_r[1]);
       u_r1[0] = ro_r1;
-      u_r1[4] = ((((sr - v_r[1]) * u_r[4] - pt_r * v_r[1]) + pt * sm) +
-                 u_hll[5] * (((vbR - sm * u_hll[5]) - vy_r1 * u_r1[6]) - vz_r1 * u_r1[7])) /
-                (sr - sm);
+      u_r1[4] = quo((((sr - v_r[1]) * u_r[4] - pt_r * v_r[1]) + pt * sm) +
+                        u_hll[5] * (((vbR - sm * u_hll[5]) - vy_r1 * u_r1[6]) - vz_r1 * u_r1[7]),
+                    sr - sm, rsr);
       u_r1[1] = ro_r1 * sm;
       u_r1[2] = ro_r1 * vy_r1;
       u_r1[3] = ro_r1 * vz_r1;
@@ -417,9 +469,9 @@ PIP_HD void hlld_flux(const double* v_l, const double* v_r, double gm, double cm
         for (int n = 5; n < 8; n++) flx[n] = sr * (u_r1[n] - v_r[n]) + flx_r[n];
       } else {
         // central states :861-898
-        double ro_ls = sqrt(ro_l1);
-        double ro_rs = sqrt(ro_r1);
-        f1 = 1.0 / (ro_ls + ro_rs);
+        double ro_ls = qsqrt(ro_l1);
+        double ro_rs = qsqrt(ro_r1);
+        f1 = qdiv(1.0, ro_ls + ro_rs);
         f2 = copysign(1.0, u_hll[5]);
         double vy_2 = f1 * ((ro_ls * vy_l1 + ro_rs * vy_r1) + (u_r1[6] - u_l1[6]) * f2);
         double vz_2 = f1 * ((ro_ls * vz_l1 + ro_rs * vz_r1) + (u_r1[7] - u_l1[7]) * f2);
@@ -545,8 +597,8 @@ PIP_HD void source_pip(double* um, double* uh, double ac, double gr, double gi, 
   cq2pv_mhd(um, pm, p);
   double nde = ph[0], nvx = ph[1], nvy = ph[2], nvz = ph[3], npr = ph[4];
   double de = pm[0], vx = pm[1], vy = pm[2], vz = pm[3], pr = pm[4];
-  double te = pr / de * p.gm * kMuP;
-  double nte = npr / nde * p.gm * kMuN;
+  double te = qdiv(pr, de) * p.gm * kMuP;
+  double nte = qdiv(npr, nde) * p.gm * kMuN;
   double dS[5];
   dS[0] = 0.0;
   if (p.pip_imp == 1) {
@@ -622,14 +674,15 @@ PIP_HD double cfl_cell(double* um, double* uh, double gmin_safety, const Phys& p
   cq2pv_mhd(um, pm, p);
   double de = pm[0], vx = pm[1], vy = pm[2], vz = pm[3], pr = pm[4];
   double bx = pm[5], by = pm[6], bz = pm[7];
-  double cs2 = p.gm * pr / dmax(de, kTiny);
-  double valf2 = ((bx * bx + by * by) + bz * bz) / dmax(de, kTiny);
+  const double dd = dmax(de, kTiny), rdd = qrcp(dd);
+  double cs2 = quo(p.gm * pr, dd, rdd);
+  double valf2 = quo((bx * bx + by * by) + bz * bz, dd, rdd);
   double vabs = sqrt((vx * vx + vy * vy) + vz * vz);
-  double dtm = gmin_safety / (vabs + sqrt(cs2 + valf2));
+  double dtm = qdiv(gmin_safety, vabs + sqrt(cs2 + valf2));
   cq2pv_hd(uh, ph, p);
-  double cs = sqrt(p.gm * ph[4] / ph[0]);
+  double cs = sqrt(qdiv(p.gm * ph[4], ph[0]));
   double vabsn = sqrt(dmax((ph[1] * ph[1] + ph[2] * ph[2]) + ph[3] * ph[3], kTiny));
-  double dth = gmin_safety / (vabsn + cs);
+  double dth = qdiv(gmin_safety, vabsn + cs);
   return dmin(dtm, dth);
 }
 

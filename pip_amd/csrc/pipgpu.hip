@@ -33,6 +33,14 @@ static int fail(int code, const std::string& msg) {
       return fail(PIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
   } while (0)
 
+// The device quotient and square root of the face phases, elementwise.
+__global__ void k_check_arith(const double* n, const double* d, double* q, double* r, long long count) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  q[k] = qdiv(n[k], d[k]);
+  r[k] = qsqrt(n[k]);
+}
+
 struct pip_ctx {
   pip_cfg cfg;
   int nv, nvt, nx, ny;
@@ -291,6 +299,28 @@ static int set_loop(pip_ctx* c, double tend, long long nmax, int active, int ion
 extern "C" {
 
 const char* pip_last_error(void) { return g_err.c_str(); }
+
+int pip_check_arith(const double* n, const double* d, double* q, double* r, long long count) {
+  if (count < 0 || (count > 0 && !(n && d && q && r))) return fail(PIP_ERR_ARG, "pip_check_arith: bad arguments");
+  if (count == 0) return PIP_OK;
+  const size_t b = (size_t)count * sizeof(double);
+  double* buf = nullptr;
+  HIPCHK(hipMalloc(&buf, 4 * b));
+  double *dn = buf, *dd = buf + count, *dq = buf + 2 * count, *dr = buf + 3 * count;
+  hipError_t e = hipMemcpy(dn, n, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dd, d, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const unsigned nb = (unsigned)((count + 255) / 256);
+    hipLaunchKernelGGL(k_check_arith, dim3(nb), dim3(256), 0, 0, dn, dd, dq, dr, count);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(q, dq, b, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(r, dr, b, hipMemcpyDeviceToHost);
+  const hipError_t ef = hipFree(buf);
+  if (e != hipSuccess) return fail(PIP_ERR_HIP, std::string("pip_check_arith: ") + hipGetErrorString(e));
+  if (ef != hipSuccess) return fail(PIP_ERR_HIP, std::string("pip_check_arith: ") + hipGetErrorString(ef));
+  return PIP_OK;
+}
 const char* pip_version(void) { return "pipgpu 0.1 (gfx950, fp64)"; }
 
 int pip_create(const pip_cfg* cfg, pip_ctx** out) {
