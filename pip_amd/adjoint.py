@@ -40,7 +40,7 @@ def _ptr(t: torch.Tensor) -> int:
 
 class TwoFluidRun:
     def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
-                 device: str = "cuda:0", save_primitives: bool = True):
+                 device: str = "cuda:0", save_primitives: bool = True, replay_tile="auto"):
         if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
             raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
         if not cfg.ac_field:
@@ -55,6 +55,14 @@ class TwoFluidRun:
         z = np.zeros((cfg.jx, cfg.ix))
         self.solver = PIPSolver(cfg, np.zeros((5, cfg.jx, cfg.ix)), np.zeros((cfg.nvar_m, cfg.jx, cfg.ix)),
                                 z, prologue=False, tile=tile)
+        # (tx, ty, xcd order, two-pass stage mask) of the recording run, and of
+        # the replays: every tile gives the same bits, and the replay kernels
+        # (no dt reduction) run faster one-pass (the default "auto")
+        t = list(tile) if tile is not None else []
+        self.tile = tuple(t[:2] + [t[2] if len(t) > 2 else None, t[3] if len(t) > 3 else 0]) if t else None
+        if isinstance(replay_tile, str) and replay_tile == "auto":
+            replay_tile = self.tile[:3] + (0,) if self.tile else None
+        self.replay_tile = tuple(replay_tile) if replay_tile is not None else None
         self.h = self.solver.h
         self.hist = torch.zeros(2 * max(self.nsteps, 1), dtype=torch.float64, device=self.device)
         self._seg_Un = None
@@ -80,6 +88,10 @@ class TwoFluidRun:
 
     def _lib_done(self):
         self.solver.sync()
+
+    def _use_tile(self, tile):
+        if tile is not None:
+            self.solver.set_tile(*tile)
 
     def _start(self, U0: torch.Tensor, ac: torch.Tensor):
         """Load (U0, ac), run the prologue (Initial_rot.f90:140-160) and the
@@ -127,7 +139,9 @@ class TwoFluidRun:
         """The same run with the recorded dt/cmax sequence held fixed (the map
         whose derivative backward() returns)."""
         self._start(U0, ac)
+        self._use_tile(self.replay_tile)
         check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(self.hist), None, None, None, None))
+        self._use_tile(self.tile)
         out = self._empty_state()
         check(lib().pip_get_state_device(self.h, _ptr(out)))
         self._lib_done()
@@ -152,6 +166,7 @@ class TwoFluidRun:
         check(L.pip_adj_seed(self.h, _ptr(lam)))
         sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
         sv = self.save_primitives
+        self._use_tile(self.replay_tile)
         for (k0, m), ck in zip(reversed(segs), reversed(ckpts)):
             check(L.pip_set_state_device(self.h, _ptr(ck)))
             check(L.pip_replay_steps(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
@@ -162,6 +177,7 @@ class TwoFluidRun:
                                      _ptr(self._seg_Vn) + sb * k if sv else None,
                                      _ptr(self._seg_V1) + sb * k if sv else None,
                                      _ptr(self.hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
+        self._use_tile(self.tile)
         self._lib_done()
         return lam, lam_ac
 

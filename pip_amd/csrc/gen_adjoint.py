@@ -188,8 +188,8 @@ class Gen:
                     out.append(f"{pad}{s} = ({e[0]}) ? {cc(e[1])} : {cc(e[2])};")
                 elif kind == "div":
                     out.append(f"{pad}{s} = PIP_VDIV({cc(e[0])}, {cc(e[1])});")
-                elif kind == "sqrt":
-                    out.append(f"{pad}{s} = sqrt({cc(e)});")
+                elif kind == "sqrt":  # constants fold through sqrt
+                    out.append(f"{pad}{s} = {'sqrt' if e.is_Number else 'PIP_VSQRT'}({cc(e)});")
                 else:
                     out.append(f"{pad}{s} = {cc(e)};")
             else:
@@ -215,7 +215,7 @@ class Gen:
             if kind == "div":
                 n, d = e
                 known = set(self.syms.values())
-                out.append(f"{pad}{{ const double q_ = PIP_VDIV({ab}, {cc(d)});")
+                out.append(f"{pad}{{ const double q_ = PIP_VRDIV({ab}, {cc(d)});")
                 if n in known:
                     out.append(f"{pad}  a_{n} += q_;")
                 if d in known:
@@ -225,7 +225,7 @@ class Gen:
                 continue
             if kind == "sqrt":
                 if e in set(self.syms.values()):
-                    out.append(f"{pad}a_{e} += PIP_VDIV(0.5 * {ab}, {s});")
+                    out.append(f"{pad}a_{e} += PIP_VRDIV(0.5 * {ab}, {s});")
                 continue
             if kind in ("min", "max"):
                 a, b = e.args
@@ -563,24 +563,31 @@ def main():
         "// forward's correctly rounded quotients, so on the device a quotient is the\n"
         "// hardware reciprocal refined by two Newton-Raphson steps\n"
         "// (r <- r + r (1 - d r)) times n, with one residual correction\n"
-        "// (q <- q + r (n - d q)): within about an ulp of n / d, in half the\n"
-        "// instructions of the IEEE division sequence and without its hazards.\n"
-        "// The VJP bodies also let the compiler contract multiply-adds.  The host\n"
-        "// build divides exactly.\n"
+        "// (q <- q + r (n - d q)) in the restated forward (PIP_VDIV: within about an\n"
+        "// ulp of n / d) and without it in the reverse sweep (PIP_VRDIV: within a\n"
+        "// few ulp); square roots take the general sequence without its range\n"
+        "// scaling (qsqrt).  The VJP bodies also let the compiler contract\n"
+        "// multiply-adds.  The host build divides exactly.\n"
         "#if defined(__HIP_DEVICE_COMPILE__)\n"
-        "__device__ __forceinline__ double pip_vdiv(double n, double d) {\n"
+        "__device__ __forceinline__ double pip_vrcp(double d) {\n"
         "  double r = __builtin_amdgcn_rcp(d);\n"
         "  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);\n"
-        "  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);\n"
+        "  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);\n"
+        "}\n"
+        "__device__ __forceinline__ double pip_vdiv(double n, double d) {\n"
+        "  const double r = pip_vrcp(d);\n"
         "  const double q = n * r;\n"
         "  return __builtin_fma(r, __builtin_fma(-d, q, n), q);\n"
         "}\n"
         "#define PIP_VDIV(n, d) pip_vdiv((n), (d))\n"
+        "#define PIP_VRDIV(n, d) ((n) * pip_vrcp(d))\n"
         "#define PIP_VJP_CONTRACT _Pragma(\"clang fp contract(fast)\")\n"
         "#else\n"
         "#define PIP_VDIV(n, d) ((n) / (d))\n"
+        "#define PIP_VRDIV(n, d) ((n) / (d))\n"
         "#define PIP_VJP_CONTRACT\n"
-        "#endif\n\n"
+        "#endif\n"
+        "#define PIP_VSQRT(x) qsqrt(x)\n\n"
         "namespace pip {\n\n",
         gen_hllc(), "\n", gen_hlld(), "\n", gen_source(), "\n}  // namespace pip\n",
     ]

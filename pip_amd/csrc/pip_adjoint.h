@@ -49,9 +49,34 @@ __global__ void __launch_bounds__(256, FL ? 2 : 1) k_adj_faces(AdjArgs a) {
   adj_face_item<FL>(a, dir, (long long)y * w + x);
 }
 
+// Interior tiles; in stage 1 (lam_in then receives the step's result) the
+// blocks past them zero the ghost ring of lam_in, adj_ghost_blocks of them.
+__host__ __device__ inline int adj_ghost_blocks(int ix, int jx) {
+  return (2 * ix + 2 * jx - 8 + 255) / 128;  // 4 ix + 4 (jx - 4) ring cells, 256 per block
+}
+
 __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
+  const int nt = adj_tiles(a.nx, a.ny);
+  if ((int)blockIdx.x >= nt) {
+    const int g = ((int)blockIdx.x - nt) * 256 + (int)threadIdx.x;
+    int i, j;
+    if (g < 4 * a.ix) {
+      const int r = g / a.ix;
+      j = r < 2 ? r : a.jx - 4 + r;
+      i = g % a.ix;
+    } else {
+      const int h = g - 4 * a.ix, c = h % 4;
+      j = 2 + h / 4;
+      i = c < 2 ? c : a.ix - 4 + c;
+      if (j >= a.jx - 2) return;
+    }
+    const long long k = (long long)j * a.ix + i;
+#pragma unroll
+    for (int n = 0; n < ADJ_NVT; n++) a.lam_in[n * a.N + k] = 0.0;
+    return;
+  }
   int x, y;
-  if (!adj_tile_item(xcd_order(blockIdx.x, gridDim.x), a.nx, a.ny, x, y)) return;
+  if (!adj_tile_item(xcd_order(blockIdx.x, nt), a.nx, a.ny, x, y)) return;
   adj_gather_item(a, (long long)(y + 2) * a.ix + (x + 2));
 }
 

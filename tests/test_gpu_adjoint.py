@@ -154,11 +154,29 @@ def test_two_pass_stage_kernels_give_the_same_gradient():
         cfg, run0, base, v0, ac, _ = _setup(48, 12, 5)
         run0.close()
         from pip_amd.adjoint import TwoFluidRun
-        run = TwoFluidRun(cfg, 12, ckpt_every=5, tile=tile)
+        run = TwoFluidRun(cfg, 12, ckpt_every=5, tile=tile, replay_tile=tile)
         U0 = build_U0(cfg, base, v0)
         out, ckpts = run.forward_record(U0, ac)
         lam, lam_ac = run.backward(torch.cos(out), ckpts, ac)
         outs.append((out, lam, lam_ac))
+        run.close()
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+def test_replay_tile_differs_from_the_recording_tile():
+    """Recording with the fluid-by-fluid kernels and replaying one-pass (the
+    default for a two-pass tile) gives the gradient of an all-one-pass run."""
+    print("case adjoint record 16x15x1x3, replay auto n=48 steps=12", flush=True)
+    from pip_amd.adjoint import TwoFluidRun
+    outs = []
+    for tile, rt in (((16, 15, 1, 0), (16, 15, 1, 0)), ((16, 15, 1, 3), "auto")):
+        cfg, run0, base, v0, ac, _ = _setup(48, 12, 5)
+        run0.close()
+        run = TwoFluidRun(cfg, 12, ckpt_every=5, tile=tile, replay_tile=rt)
+        assert run.replay_tile == (16, 15, 1, 0)
+        out, ckpts = run.forward_record(build_U0(cfg, base, v0), ac)
+        outs.append((out,) + run.backward(torch.cos(out), ckpts, ac))
         run.close()
     for a, b in zip(*outs):
         assert torch.equal(a, b)
