@@ -125,7 +125,8 @@ def cpu_baseline(n: int, steps: int):
 def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000):
     """Ensemble of n_members 1-D Brio-Wu-type shock tubes (nx log-uniform in
     [256, 4096], xi_n, alpha_c and the pressure ratio drawn from seed 7) run
-    to t = tend: one persistent launch vs sequential plain launches."""
+    to t = tend: the batched ensemble (launch mode and width autotuned) vs
+    sequential plain launches."""
     import torch
     from pip_amd import autotune, problems
     from pip_amd.ensemble import Ensemble
@@ -133,9 +134,9 @@ def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000)
     params = problems.ensemble_members(n_members, seed=7)
     mem = [problems.ensemble_member(p)[:3] for p in params]
     ens = Ensemble(mem, tend=tend, max_steps=max_steps)
-    tile, tune_ms = autotune.tune_ensemble(ens, f"ens:{n_members}:seed7:tend{tend}")
+    (mode, tile), tune_ms = autotune.tune_ensemble(ens, f"ens:{n_members}:seed7:tend{tend}")
     ens.reset()
-    ens.run(graph=True)  # builds and caches the graph
+    ens.run(graph=True)  # builds and caches the graph (persistent mode)
     ens.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -159,7 +160,7 @@ def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000)
         s.close()
     ens.close()
     return {"workload": f"{n_members} x 1-D two-fluid shock tube, nx log-uniform [256,4096], t_end={tend}",
-            "cell_updates": updates, "tile": tile, "tile_timings_ms": tune_ms,
+            "cell_updates": updates, "mode": mode, "tile": tile, "tile_timings_ms": tune_ms,
             "ensemble_s": wall_ens, "ensemble_device_ms": ens_ms, "sequential_plain_s": wall_seq,
             "ensemble_value": updates / wall_ens, "sequential_value": updates / wall_seq,
             "unit": "cell-updates/s", "speedup": wall_seq / wall_ens, "bitwise_identical": bool(identical)}

@@ -102,6 +102,7 @@ _SIGS = {
     "pip_ens_upload": [_V, _I, _P, _P],
     "pip_ens_download": [_V, _I, _P, _P],
     "pip_ens_set_tile": [_V, _I],
+    "pip_ens_set_mode": [_V, _I],
     "pip_ens_reset": [_V],
     "pip_ens_run": [_V, _I, _PD],
     "pip_ens_result": [_V, _I, _PD, _PL, _PI],

@@ -1,5 +1,5 @@
 """Autotuner for the tile shapes of the stage kernels and the ensemble
-workgroup width.  Each candidate is timed on the real problem (device time
+launch mode and workgroup width.  Each candidate is timed on the real problem (device time
 from hipEvents) and the fastest is cached in pip_amd/lib/autotune.json,
 keyed by problem signature; results are identical for every candidate, so
 the choice affects speed only."""
@@ -15,6 +15,7 @@ STAGE_TILES = [(16, 15), (32, 7), (32, 15), (64, 7), (32, 8), (16, 7)]
 XCD_REMAP = [0, 1]  # row-major tile order, or contiguous tile ranges per XCD
 TWO_PASS_TILES = [(16, 15), (32, 7), (16, 7)]  # tiles of the fluid-by-fluid stage kernel
 ENS_TILES = [64, 128, 256, 512]
+ENS_MODES = ["persistent", "lockstep"]  # one launch, or every member spread over workgroups
 
 
 def _load():
@@ -70,18 +71,26 @@ def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True):
 
 
 def tune_ensemble(ens, key: str, use_cache: bool = True):
-    """Time the whole ensemble for each workgroup width; returns (best, timings)."""
+    """Time the whole ensemble for each launch mode and workgroup width;
+    sets the fastest and returns ((mode, width), timings)."""
     cache = _load()
+    key = key + ":modes"
     if use_cache and key in cache:
-        ens.set_tile(cache[key]["tile"])
-        return cache[key]["tile"], cache[key]["ms"]
-    timings = {}
-    for tx in ENS_TILES:
+        mode, tx = cache[key]["best"]
+        ens.set_mode(mode)
         ens.set_tile(tx)
-        ens.reset()
-        timings[str(tx)] = ens.run(graph=False)
-    best = int(min(timings, key=timings.get))
-    ens.set_tile(best)
-    cache[key] = {"tile": best, "ms": timings}
+        return (mode, tx), cache[key]["ms"]
+    timings = {}
+    for mode in ENS_MODES:
+        ens.set_mode(mode)
+        for tx in ENS_TILES:
+            ens.set_tile(tx)
+            ens.reset()
+            timings[f"{mode}:{tx}"] = ens.run(graph=False)
+    best = min(timings, key=timings.get)
+    mode, tx = best.split(":")[0], int(best.split(":")[1])
+    ens.set_mode(mode)
+    ens.set_tile(tx)
+    cache[key] = {"best": [mode, tx], "ms": timings}
     _save(cache)
-    return best, timings
+    return (mode, tx), timings

@@ -914,6 +914,60 @@ __device__ __forceinline__ double ens_dt(const RoundOut& o, double safety) {
   return dmin(dttemp, dmin(safety, 0.3) / o.rhoac);
 }
 
+// StageArgs of ensemble member M (1-D, nvar_m = 8, constant alpha_c,
+// explicit collisions, no IR); the state pointers are set per stage.
+__device__ __forceinline__ StageArgs ens_stage_args(const EnsArgs& e, const EnsMember& M) {
+  StageArgs a;
+  a.ac = nullptr;
+  a.ac_out = nullptr;
+  a.U0out = nullptr;
+  a.xcd_remap = 0;
+  a.arb_heat = nullptr;
+  a.ion_pot_out = nullptr;
+  a.Vout = nullptr;
+  a.dtc = nullptr;
+  a.sc = nullptr;
+  a.ix = M.ix;
+  a.jx = 1;
+  a.nx = M.nx;
+  a.ny = 1;
+  a.N = M.ix;
+  a.t_order = 2;
+  a.col = M.col;
+  a.gmin_safety = M.gmin_safety;
+  a.dxyz_min = M.dx;
+  a.dx = M.dx;
+  a.dy = 100.0;
+  a.db_clean = e.db_clean;
+  a.ph = e.ph;
+  a.irc = IRConst{0, 0, 0, 0, 0, 0, 0};
+  a.bnd = e.bnd;
+  return a;
+}
+
+// Start of a member's run by one workgroup: the boundary map of the
+// initial state (Initial_rot.f90:160), the prologue cfl and the loop-top
+// cfl of the first iteration; returns the first dt.
+template <int TX>
+__device__ __forceinline__ double ens_start(const EnsArgs& e, const EnsMember& M, const StageArgs& a,
+                                            double* U, double* lds_red) {
+  if (threadIdx.x < 4) {
+    int g = (threadIdx.x < 2) ? threadIdx.x : M.ix - 4 + threadIdx.x;
+    bool flip;
+    int sidx = src_index(g, M.ix, e.bnd.type[0], e.bnd.type[1], flip);
+    unsigned msk = flip ? e.bnd.sym[(g < 2) ? 0 : 1] : 0u;
+    for (int n = 0; n < 13; n++) {
+      double v = U[n * a.N + sidx];
+      U[n * a.N + g] = ((msk >> n) & 1u) ? -v : v;
+    }
+  }
+  __syncthreads();
+  ens_cfl_cells<TX>(U, a, lds_red);
+  __syncthreads();
+  RoundOut o = ens_cfl_cells<TX>(U, a, lds_red);
+  return ens_dt(o, e.ph.safety);
+}
+
 template <int TX>
 __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
   extern __shared__ double smem[];
@@ -931,48 +985,8 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
     const EnsMember M = e.mem[m];
     double* U = e.U + M.off;
     double* U1 = e.U1 + M.off;
-    StageArgs a;
-    a.ac = nullptr;
-    a.ac_out = nullptr;
-    a.U0out = nullptr;
-    a.xcd_remap = 0;
-    a.arb_heat = nullptr;
-    a.ion_pot_out = nullptr;
-    a.Vout = nullptr;
-    a.dtc = nullptr;
-    a.sc = nullptr;
-    a.ix = M.ix;
-    a.jx = 1;
-    a.nx = M.nx;
-    a.ny = 1;
-    a.N = M.ix;
-    a.t_order = 2;
-    a.col = M.col;
-    a.gmin_safety = M.gmin_safety;
-    a.dxyz_min = M.dx;
-    a.dx = M.dx;
-    a.dy = 100.0;
-    a.db_clean = e.db_clean;
-    a.ph = e.ph;
-    a.irc = IRConst{0, 0, 0, 0, 0, 0, 0};
-    a.bnd = e.bnd;
-    // prologue: boundary map of the initial state (Initial_rot.f90:160)
-    if (threadIdx.x < 4) {
-      int g = (threadIdx.x < 2) ? threadIdx.x : M.ix - 4 + threadIdx.x;
-      bool flip;
-      int sidx = src_index(g, M.ix, e.bnd.type[0], e.bnd.type[1], flip);
-      unsigned msk = flip ? e.bnd.sym[(g < 2) ? 0 : 1] : 0u;
-      for (int n = 0; n < 13; n++) {
-        double v = U[n * a.N + sidx];
-        U[n * a.N + g] = ((msk >> n) & 1u) ? -v : v;
-      }
-    }
-    __syncthreads();
-    // prologue cfl, then the loop-top cfl of the first iteration
-    ens_cfl_cells<TX>(U, a, lds_red);
-    __syncthreads();
-    RoundOut o = ens_cfl_cells<TX>(U, a, lds_red);
-    double dt = ens_dt(o, e.ph.safety);
+    StageArgs a = ens_stage_args(e, M);
+    double dt = ens_start<TX>(e, M, a, U, lds_red);
     double time = 0.0;
     long long nt = 0;
     int status = 0;
@@ -1025,6 +1039,111 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
       out.pad = 0;
       e.out[m] = out;
     }
+  }
+}
+
+// ------------------------------------------------------ lockstep ensemble
+// The ensemble with every member spread over workgroups instead of one: a
+// start kernel (one workgroup per member), then per step two stage launches
+// over a table of (member, tile) pairs of the running members and a step
+// end with one thread per member.  Per-member dt, time, step count and
+// reductions live in an array of DevScalars.  Every operation is the
+// persistent loop's (k_ensemble) and the plain path's, so the results agree
+// with both bit for bit; the members' tiles run in parallel, as in the plain
+// path, instead of one after another on one CU.
+struct LockTile {
+  int m, i0;  // member, first cell of the tile
+};
+
+template <int TX>
+__global__ void __launch_bounds__(TX) k_lock_start(EnsArgs e, DevScalars* scs) {
+  extern __shared__ double smem[];
+  constexpr int TC = TX - 1;
+  using G = TileGeom<1, 8, TC, 1>;
+  double* lds_red = smem + G::LDS_DOUBLES;
+  const int m = blockIdx.x;
+  if (m >= e.n_members) return;
+  const EnsMember M = e.mem[m];
+  const StageArgs a = ens_stage_args(e, M);
+  const double dt = ens_start<TX>(e, M, a, e.U + M.off, lds_red);
+  if (threadIdx.x == 0) {
+    DevScalars* sc = scs + m;
+    sc->dt = dt;
+    sc->time = 0.0;
+    sc->cmax = 0.0;
+    sc->nt = 0;
+    sc->status = 0;
+    sc->red[RED_DT] = as_u64(1.0e300);
+    sc->red[RED_RHOAC] = 0ull;
+    sc->red[RED_GMAX] = 0ull;
+    sc->red[RED_IONPOT] = 0ull;
+    sc->red[RED_NAN] = 0ull;
+    const bool go = (0.0 < M.tend) && (0 < M.nmax);
+    sc->active = go ? 1 : 0;
+    sc->last = ((0.0 + dt >= M.tend) || (1 >= M.nmax) || (dt <= 1.0e-12)) ? 1 : 0;
+    if (!go) e.out[m] = EnsOut{0.0, dt, 0, 0, 0};
+  }
+}
+
+// Stage ISTEP (1 or 2) of the running members, one tile per workgroup;
+// stage 2 carries the cfl rounds and reduces into the member's slots.
+template <int TX, int ISTEP>
+__global__ void __launch_bounds__(TX) k_lock_stage(EnsArgs e, DevScalars* scs, const LockTile* tiles) {
+  extern __shared__ double smem[];
+  constexpr int TC = TX - 1;
+  const LockTile t = tiles[blockIdx.x];
+  DevScalars* sc = scs + t.m;
+  if (!sc->active) return;
+  const EnsMember M = e.mem[t.m];
+  StageArgs a = ens_stage_args(e, M);
+  double* U = e.U + M.off;
+  double* U1 = e.U1 + M.off;
+  a.istep = ISTEP;
+  a.Uin = (ISTEP == 1) ? U : U1;
+  a.U0 = U;
+  a.Uout = (ISTEP == 1) ? U1 : U;
+  a.epi = 0;
+  const int epi = (ISTEP == 1) ? 0 : (sc->last ? 1 : 2);
+  RoundOut r;
+  int nanflag;
+  const bool valid = stage_tile<1, 8, false, false, TC, 1, TX>(a, sc->dt, 0.0, epi, t.i0, 0, smem, r, nanflag);
+  if (ISTEP == 2) {
+    __syncthreads();
+    block_reduce_store<TX>(r, nanflag, valid, sc, smem);
+  }
+}
+
+// Step end of every running member: the persistent loop's dt, time, step
+// count and status update; a member that stops writes its EnsOut.
+__global__ void k_lock_end(EnsArgs e, DevScalars* scs) {
+  const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (m >= e.n_members) return;
+  DevScalars* sc = scs + m;
+  if (!sc->active) return;
+  const EnsMember M = e.mem[m];
+  const RoundOut red = {as_f64(sc->red[RED_DT]), as_f64(sc->red[RED_RHOAC]), 0.0, 0.0};
+  const bool fl = sc->red[RED_NAN] != 0ull;
+  sc->red[RED_DT] = as_u64(1.0e300);
+  sc->red[RED_RHOAC] = 0ull;
+  sc->red[RED_GMAX] = 0ull;
+  sc->red[RED_IONPOT] = 0ull;
+  sc->red[RED_NAN] = 0ull;
+  const double dt = sc->dt;
+  const double dt_new = ens_dt(red, e.ph.safety);
+  const double time = sc->time + dt;
+  const long long nt = sc->nt + 1;
+  int status = sc->status;
+  if (dt <= 1.0e-12) status |= 1;
+  if (fl) status |= 2;
+  sc->time = time;
+  sc->nt = nt;
+  sc->status = status;
+  if (time < M.tend && nt < M.nmax && status == 0) {
+    sc->dt = dt_new;
+    sc->last = ((time + dt_new >= M.tend) || (nt + 1 >= M.nmax) || (dt_new <= 1.0e-12)) ? 1 : 0;
+  } else {
+    sc->active = 0;
+    e.out[m] = EnsOut{time, dt, nt, status, 0};
   }
 }
 }  // namespace pip

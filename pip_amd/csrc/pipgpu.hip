@@ -927,6 +927,12 @@ struct pip_ens {
   Bnd bnd;
   hipGraphExec_t gexec = nullptr;
   int graph_tx = 0;
+  int mode = 0;                     // 0: persistent launch, 1: lockstep launches
+  DevScalars* scs = nullptr;        // lockstep: per-member scalars
+  LockTile* tiles = nullptr;        // lockstep: (member, tile) table of the running members
+  LockTile* h_tiles = nullptr;      // pinned staging of the table
+  DevScalars* h_scs = nullptr;      // pinned copy of the scalars (which members still run)
+  int max_tiles = 0;
 };
 
 template <int TX>
@@ -961,6 +967,75 @@ static int ens_launch(pip_ens* e) {
   hipLaunchKernelGGL(kern, dim3(grid), dim3(TX), lds, e->stream, a);
   HIPCHK(hipGetLastError());
   return PIP_OK;
+}
+
+// Lockstep run: start kernel, then chunks of steps over the tile table of
+// the members still running; the table is rebuilt between chunks from the
+// members' active flags (one 64-step chunk per host round trip).
+template <int TX>
+static int ens_lockstep_t(pip_ens* e) {
+  constexpr int TC = TX - 1;
+  using G = TileGeom<1, 8, TC, 1>;
+  const size_t lds_stage = sizeof(double) * (size_t)G::LDS_DOUBLES;
+  const size_t lds_start = sizeof(double) * (size_t)(G::LDS_DOUBLES + 3 * (TX / 64));
+  static bool attr = false;
+  if (!attr) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_start<TX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_start));
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    attr = true;
+  }
+  EnsArgs a;
+  a.U = e->U;
+  a.U1 = e->U1;
+  a.mem = e->mem;
+  a.out = e->out;
+  a.order = e->order;
+  a.counter = e->counter;
+  a.n_members = e->n;
+  a.ph = e->ph;
+  a.bnd = e->bnd;
+  a.db_clean = e->base.db_clean;
+  hipLaunchKernelGGL(k_lock_start<TX>, dim3((unsigned)e->n), dim3(TX), lds_start, e->stream, a, e->scs);
+  HIPCHK(hipGetLastError());
+  std::vector<char> run(e->n, 1);  // members that may still run (all, until the start kernel has said)
+  const unsigned gend = (unsigned)((e->n + 255) / 256);
+  const int chunk = 64;
+  for (;;) {
+    int nt = 0;
+    for (int m = 0; m < e->n; m++) {
+      if (!run[m]) continue;
+      for (int i0 = 0; i0 < e->hmem[m].nx; i0 += TC) e->h_tiles[nt++] = LockTile{m, i0};
+    }
+    if (nt == 0) break;
+    if (nt > e->max_tiles) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
+    HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
+    for (int k = 0; k < chunk; k++) {
+      hipLaunchKernelGGL((k_lock_stage<TX, 1>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
+                         e->tiles);
+      hipLaunchKernelGGL((k_lock_stage<TX, 2>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
+                         e->tiles);
+      hipLaunchKernelGGL(k_lock_end, dim3(gend), dim3(256), 0, e->stream, a, e->scs);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(e->h_scs, e->scs, sizeof(DevScalars) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int m = 0; m < e->n; m++) run[m] = (char)(e->h_scs[m].active != 0);
+  }
+  return PIP_OK;
+}
+
+static int ens_lockstep(pip_ens* e) {
+  switch (e->tx) {
+    case 64: return ens_lockstep_t<64>(e);
+    case 128: return ens_lockstep_t<128>(e);
+    case 256: return ens_lockstep_t<256>(e);
+    case 512: return ens_lockstep_t<512>(e);
+  }
+  return fail(PIP_ERR_ARG, "ensemble tile must be 64, 128, 256 or 512");
 }
 
 static int ens_enqueue(pip_ens* e) {
@@ -1047,6 +1122,13 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   if ((er = hipMalloc(&e->out, sizeof(EnsOut) * n)) != hipSuccess) return bail("hipMalloc out");
   if ((er = hipMalloc(&e->order, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc order");
   if ((er = hipMalloc(&e->counter, sizeof(unsigned) * 4)) != hipSuccess) return bail("hipMalloc counter");
+  for (int m = 0; m < n; m++) e->max_tiles += (members[m].nx + 62) / 63;  // tiles of the narrowest width
+  if ((er = hipMalloc(&e->scs, sizeof(DevScalars) * n)) != hipSuccess) return bail("hipMalloc scalars");
+  if ((er = hipMalloc(&e->tiles, sizeof(LockTile) * e->max_tiles)) != hipSuccess) return bail("hipMalloc tiles");
+  if ((er = hipHostMalloc((void**)&e->h_tiles, sizeof(LockTile) * e->max_tiles, hipHostMallocDefault)) != hipSuccess)
+    return bail("hipHostMalloc tiles");
+  if ((er = hipHostMalloc((void**)&e->h_scs, sizeof(DevScalars) * n, hipHostMallocDefault)) != hipSuccess)
+    return bail("hipHostMalloc scalars");
   if ((er = hipMemcpy(e->mem, e->hmem.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
   if ((er = hipMemcpy(e->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
   if ((er = hipMemset(e->U, 0, sizeof(double) * off)) != hipSuccess) return bail("memset");
@@ -1068,6 +1150,10 @@ int pip_ens_destroy(pip_ens* e) {
   (void)hipFree(e->out);
   (void)hipFree(e->order);
   (void)hipFree(e->counter);
+  (void)hipFree(e->scs);
+  (void)hipFree(e->tiles);
+  if (e->h_tiles) (void)hipHostFree(e->h_tiles);
+  if (e->h_scs) (void)hipHostFree(e->h_scs);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PIP_OK;
@@ -1100,6 +1186,13 @@ int pip_ens_set_tile(pip_ens* e, int tx) {
   return PIP_OK;
 }
 
+int pip_ens_set_mode(pip_ens* e, int mode) {
+  if (!e) return fail(PIP_ERR_ARG, "null ensemble");
+  if (mode != 0 && mode != 1) return fail(PIP_ERR_ARG, "ensemble mode must be 0 (persistent) or 1 (lockstep)");
+  e->mode = mode;
+  return PIP_OK;
+}
+
 int pip_ens_reset(pip_ens* e) {
   if (!e) return fail(PIP_ERR_ARG, "null ensemble");
   HIPCHK(hipMemcpyAsync(e->U, e->Uinit, sizeof(double) * e->total, hipMemcpyDeviceToDevice, e->stream));
@@ -1112,7 +1205,12 @@ int pip_ens_run(pip_ens* e, int use_graph, double* ms) {
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
-  if (use_graph) {
+  if (e->mode == 1) {  // host-driven chunks: no graph
+    HIPCHK(hipEventRecord(a, e->stream));
+    int rc = ens_lockstep(e);
+    if (rc != PIP_OK) return rc;
+    HIPCHK(hipEventRecord(b, e->stream));
+  } else if (use_graph) {
     if (!e->gexec || e->graph_tx != e->tx) {
       if (e->gexec) {
         HIPCHK(hipGraphExecDestroy(e->gexec));

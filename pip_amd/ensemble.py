@@ -58,6 +58,7 @@ class Ensemble:
         for k, (cfg, uh, um) in enumerate(members):
             check(lib().pip_ens_upload(self.h, k, _p(np.ascontiguousarray(uh)), _p(np.ascontiguousarray(um))))
         self.set_tile(tile)
+        self.mode = "persistent"
 
     def close(self):
         if getattr(self, "h", None):
@@ -73,6 +74,13 @@ class Ensemble:
     def set_tile(self, tx: int):
         check(lib().pip_ens_set_tile(self.h, int(tx)))
         self.tile = tx
+
+    def set_mode(self, mode: str):
+        """'persistent': one launch, a workgroup per member (hipGraph replay
+        with graph=True); 'lockstep': every member's tiles spread over
+        workgroups, two stage launches and a step end per step."""
+        check(lib().pip_ens_set_mode(self.h, {"persistent": 0, "lockstep": 1}[mode]))
+        self.mode = mode
 
     def reset(self):
         check(lib().pip_ens_reset(self.h))

@@ -1,5 +1,6 @@
-"""Batched ensemble (one persistent launch, optional hipGraph replay) equals
-the plain per-member runs bit for bit."""
+"""Batched ensemble (one persistent launch, optional hipGraph replay, or
+lockstep launches with every member spread over workgroups) equals the plain
+per-member runs bit for bit."""
 import numpy as np
 import pytest
 
@@ -26,11 +27,14 @@ def _members(n, nx_cap=None):
 MAXS = 20000
 
 
-@pytest.mark.parametrize("graph,tile", [(True, 256), (False, 64), (True, 512)])
-def test_ensemble_matches_plain_runs(graph, tile):
-    print("case ensemble 10 members graph", graph, "tile", tile, flush=True)
+@pytest.mark.parametrize("mode,graph,tile", [("persistent", True, 256), ("persistent", False, 64),
+                                             ("persistent", True, 512), ("lockstep", False, 256),
+                                             ("lockstep", False, 64), ("lockstep", False, 512)])
+def test_ensemble_matches_plain_runs(mode, graph, tile):
+    print("case ensemble 10 members mode", mode, "graph", graph, "tile", tile, flush=True)
     mem = _members(10)
     ens = Ensemble(mem, tend=0.1, max_steps=MAXS, tile=tile)
+    ens.set_mode(mode)
     ms = ens.run(graph=graph)
     print("ensemble ms", ms, flush=True)
     for k, (cfg, uh, um) in enumerate(mem):
@@ -57,3 +61,24 @@ def test_ensemble_rerun_after_reset_is_identical():
     for k in range(6):
         b = ens.state(k)
         assert np.array_equal(a[k][0], b[0]) and np.array_equal(a[k][1], b[1])
+
+
+def test_lockstep_equals_persistent_at_the_step_cap():
+    """Lockstep and persistent launches give the same states (ghosts
+    included), times, step counts and statuses when the members stop at the
+    step cap inside a 64-step chunk, and again after a reset."""
+    print("case ensemble lockstep vs persistent, 12 members, cap 150 steps", flush=True)
+    mem = _members(12, nx_cap=1500)
+    ens = Ensemble(mem, tend=0.1, max_steps=150, tile=128)
+    ens.run(graph=False)
+    ref = [(ens.result(k), ens.state(k)) for k in range(12)]
+    assert all(r["nt"] == 150 for r, _ in ref)
+    ens.set_mode("lockstep")
+    for _ in range(2):
+        ens.reset()
+        ens.run()
+        for k in range(12):
+            r, (uh, um) = ref[k]
+            assert ens.result(k) == r, k
+            eh, em = ens.state(k)
+            assert np.array_equal(eh, uh) and np.array_equal(em, um), k
