@@ -166,6 +166,27 @@ def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000)
             "unit": "cell-updates/s", "speedup": wall_seq / wall_ens, "bitwise_identical": bool(identical)}
 
 
+def ensemble_leg(mode: str, tile: int, n_members: int = 256, tend: float = 0.1, max_steps: int = 20000):
+    """The batched leg's ensemble alone in one launch mode and width, run
+    twice from its initial state (for profiles)."""
+    from pip_amd import problems
+    from pip_amd.ensemble import Ensemble
+    params = problems.ensemble_members(n_members, seed=7)
+    mem = [problems.ensemble_member(p)[:3] for p in params]
+    ens = Ensemble(mem, tend=tend, max_steps=max_steps, tile=tile)
+    ens.set_mode(mode)
+    runs = []
+    for _ in range(2):
+        ens.reset()
+        t0 = time.perf_counter()
+        ms = ens.run(graph=True)
+        runs.append({"device_ms": ms, "wall_s": time.perf_counter() - t0})
+    steps = max(ens.result(k)["nt"] for k in range(n_members))
+    ens.close()
+    return {"workload": f"{n_members} x 1-D two-fluid shock tube, seed 7, t_end={tend}", "mode": mode,
+            "tile": tile, "longest_run_steps": steps, "runs": runs}
+
+
 def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     """Gradient of J = |U_T|^2/2 (interior) w.r.t. the initial state and the
     per-cell alpha_c field vs the plain forward run (pip_advance)."""
@@ -225,9 +246,12 @@ def main():
     ap.add_argument("--pmc-summary", metavar="DIR", help="write the HBM-traffic profile summary from "
                     "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write; the main "
                     "runs used --tile) and exit")
-    ap.add_argument("--only", choices=["batched", "adjoint", "copy"],
+    ap.add_argument("--only", choices=["batched", "adjoint", "copy", "ensemble"],
                     help="run one extra leg alone (profiling); copy = 1 GiB device copy that "
-                         "calibrates the HBM byte counters")
+                         "calibrates the HBM byte counters; ensemble = the batched leg's ensemble "
+                         "alone, twice, in --ens-mode at --ens-tile (no autotuning, no plain runs)")
+    ap.add_argument("--ens-mode", choices=["persistent", "lockstep"], default="lockstep")
+    ap.add_argument("--ens-tile", type=int, choices=[64, 128, 256, 512], default=256)
     args = ap.parse_args()
     if args.pmc_summary:
         pm = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
@@ -256,6 +280,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n}}))
+        return
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_leg(args.ens_mode, args.ens_tile)}))
         return
     if args.only:
         print(json.dumps({args.only: batched_leg() if args.only == "batched" else adjoint_leg()}))
