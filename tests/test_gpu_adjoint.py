@@ -13,7 +13,7 @@ wave and larger than the second differences the minmod limiter compares
 (~1e-7 at 512 cells per wavelength), so the difference quotient would
 straddle limiter switches instead of measuring the derivative.  J(+) - J(-) is summed as
 sum (U+ - U-) . ((U+ + U-)/2 - U*), which is exact in the cells the
-perturbation does not reach.  Agreement is judged on the 64-cell vectors:
+perturbation does not reach.  Agreement is measured on the 64-cell vectors:
 the relative L2 error ||g_fd - g_adj|| / ||g_fd|| and the median per-cell
 relative error (a cell whose gradient is near zero has a per-cell ratio
 dominated by the round-off of the two replays, not by the adjoint).
