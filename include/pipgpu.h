@@ -181,9 +181,10 @@ int pip_ens_download(pip_ens *e, int m, double *U_h, double *U_m);
 int pip_ens_set_tile(pip_ens *e, int tx);
 /* 0: one persistent launch, a workgroup per member (default; use_graph
  * replays it from a hipGraph); 1: lockstep, each member's tiles spread over
- * workgroups, two stage launches and a step end per step for all running
- * members, host-driven in 64-step chunks (use_graph ignored).  Both give
- * every member's plain pip_advance run bit for bit. */
+ * workgroups, two stage launches per step for all running members (stage 1
+ * applies the previous step's end), host-driven in 64-step chunks
+ * (use_graph ignored).  Both give every member's plain pip_advance run bit
+ * for bit. */
 int pip_ens_set_mode(pip_ens *e, int mode);
 /* restore every member to its uploaded state (device copy, not timed) */
 int pip_ens_reset(pip_ens *e);

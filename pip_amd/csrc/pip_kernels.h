@@ -1045,15 +1045,53 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
 // ------------------------------------------------------ lockstep ensemble
 // The ensemble with every member spread over workgroups instead of one: a
 // start kernel (one workgroup per member), then per step two stage launches
-// over a table of (member, tile) pairs of the running members and a step
-// end with one thread per member.  Per-member dt, time, step count and
-// reductions live in an array of DevScalars.  Every operation is the
-// persistent loop's (k_ensemble) and the plain path's, so the results agree
-// with both bit for bit; the members' tiles run in parallel, as in the plain
-// path, instead of one after another on one CU.
+// over a table of (member, tile) pairs of the running members.  Per-member
+// dt, time, step count, flags and reductions live in DevScalars, two
+// buffers by step parity: stage 1 of step k applies the end of step k-1
+// (read from buffer (k-1)&1, in every workgroup of the member alike) and the
+// member's first tile writes the result to buffer k&1, which stage 2 then
+// reads and reduces into; at the end of a host chunk, k_lock_end applies
+// the last step's end.  Every operation is the persistent loop's
+// (k_ensemble) and the plain path's, so the results agree with both bit for
+// bit; the members' tiles run in parallel, as in the plain path, instead of
+// one after another on one CU.
 struct LockTile {
   int m, i0;  // member, first cell of the tile
 };
+
+// The persistent loop's step end for one member: dt of the next step from
+// the reductions, time, step count, status, the active / last flags, and
+// reset slots.  A member that has stopped keeps its state; *stops says
+// whether it stops at this step end.
+__device__ __forceinline__ DevScalars lock_step_end(const DevScalars& sc, const EnsMember& M, double safety,
+                                                   bool* stops) {
+  DevScalars o = sc;
+  *stops = false;
+  if (!sc.active) return o;
+  const RoundOut red = {as_f64(sc.red[RED_DT]), as_f64(sc.red[RED_RHOAC]), 0.0, 0.0};
+  const bool fl = sc.red[RED_NAN] != 0ull;
+  o.red[RED_DT] = as_u64(1.0e300);
+  o.red[RED_RHOAC] = 0ull;
+  o.red[RED_GMAX] = 0ull;
+  o.red[RED_IONPOT] = 0ull;
+  o.red[RED_NAN] = 0ull;
+  const double dt = sc.dt;
+  const double dt_new = ens_dt(red, safety);
+  o.time = sc.time + dt;
+  o.nt = sc.nt + 1;
+  int status = sc.status;
+  if (dt <= 1.0e-12) status |= 1;
+  if (fl) status |= 2;
+  o.status = status;
+  if (o.time < M.tend && o.nt < M.nmax && status == 0) {
+    o.dt = dt_new;
+    o.last = ((o.time + dt_new >= M.tend) || (o.nt + 1 >= M.nmax) || (dt_new <= 1.0e-12)) ? 1 : 0;
+  } else {
+    o.active = 0;
+    *stops = true;
+  }
+  return o;
+}
 
 template <int TX>
 __global__ void __launch_bounds__(TX) k_lock_start(EnsArgs e, DevScalars* scs) {
@@ -1067,34 +1105,58 @@ __global__ void __launch_bounds__(TX) k_lock_start(EnsArgs e, DevScalars* scs) {
   const StageArgs a = ens_stage_args(e, M);
   const double dt = ens_start<TX>(e, M, a, e.U + M.off, lds_red);
   if (threadIdx.x == 0) {
-    DevScalars* sc = scs + m;
-    sc->dt = dt;
-    sc->time = 0.0;
-    sc->cmax = 0.0;
-    sc->nt = 0;
-    sc->status = 0;
-    sc->red[RED_DT] = as_u64(1.0e300);
-    sc->red[RED_RHOAC] = 0ull;
-    sc->red[RED_GMAX] = 0ull;
-    sc->red[RED_IONPOT] = 0ull;
-    sc->red[RED_NAN] = 0ull;
+    DevScalars sc;
+    sc.dt = dt;
+    sc.time = 0.0;
+    sc.cmax = 0.0;
+    sc.tend = M.tend;
+    sc.dt_out = dt;
+    sc.nt = 0;
+    sc.nmax = M.nmax;
+    sc.status = 0;
+    sc.ion_pot_valid = 0;
+    sc.red[RED_DT] = as_u64(1.0e300);
+    sc.red[RED_RHOAC] = 0ull;
+    sc.red[RED_GMAX] = 0ull;
+    sc.red[RED_IONPOT] = 0ull;
+    sc.red[RED_NAN] = 0ull;
     const bool go = (0.0 < M.tend) && (0 < M.nmax);
-    sc->active = go ? 1 : 0;
-    sc->last = ((0.0 + dt >= M.tend) || (1 >= M.nmax) || (dt <= 1.0e-12)) ? 1 : 0;
+    sc.active = go ? 1 : 0;
+    sc.last = ((0.0 + dt >= M.tend) || (1 >= M.nmax) || (dt <= 1.0e-12)) ? 1 : 0;
+    scs[m] = sc;  // both parity buffers
+    scs[e.n_members + m] = sc;
     if (!go) e.out[m] = EnsOut{0.0, dt, 0, 0, 0};
   }
 }
 
-// Stage ISTEP (1 or 2) of the running members, one tile per workgroup;
-// stage 2 carries the cfl rounds and reduces into the member's slots.
+// Stage ISTEP (1 or 2) of step k of the running members, one tile per
+// workgroup; p = k & 1.  Stage 1 with apply_end first applies step k-1's
+// end; stage 2 carries the cfl rounds and reduces into the member's slots.
 template <int TX, int ISTEP>
-__global__ void __launch_bounds__(TX) k_lock_stage(EnsArgs e, DevScalars* scs, const LockTile* tiles) {
+__global__ void __launch_bounds__(TX) k_lock_stage(EnsArgs e, DevScalars* scs, const LockTile* tiles, int p,
+                                                   int apply_end) {
   extern __shared__ double smem[];
   constexpr int TC = TX - 1;
   const LockTile t = tiles[blockIdx.x];
-  DevScalars* sc = scs + t.m;
-  if (!sc->active) return;
   const EnsMember M = e.mem[t.m];
+  DevScalars* sc = scs + (long long)p * e.n_members + t.m;
+  double dt;
+  int last;
+  if (ISTEP == 1 && apply_end) {
+    bool stops;
+    const DevScalars nw = lock_step_end(scs[(long long)(p ^ 1) * e.n_members + t.m], M, e.ph.safety, &stops);
+    if (t.i0 == 0 && threadIdx.x == 0) {
+      *sc = nw;
+      if (stops) e.out[t.m] = EnsOut{nw.time, nw.dt, nw.nt, nw.status, 0};
+    }
+    if (!nw.active) return;
+    dt = nw.dt;
+    last = nw.last;
+  } else {
+    if (!sc->active) return;
+    dt = sc->dt;
+    last = sc->last;
+  }
   StageArgs a = ens_stage_args(e, M);
   double* U = e.U + M.off;
   double* U1 = e.U1 + M.off;
@@ -1103,47 +1165,26 @@ __global__ void __launch_bounds__(TX) k_lock_stage(EnsArgs e, DevScalars* scs, c
   a.U0 = U;
   a.Uout = (ISTEP == 1) ? U1 : U;
   a.epi = 0;
-  const int epi = (ISTEP == 1) ? 0 : (sc->last ? 1 : 2);
+  const int epi = (ISTEP == 1) ? 0 : (last ? 1 : 2);
   RoundOut r;
   int nanflag;
-  const bool valid = stage_tile<1, 8, false, false, TC, 1, TX>(a, sc->dt, 0.0, epi, t.i0, 0, smem, r, nanflag);
+  const bool valid = stage_tile<1, 8, false, false, TC, 1, TX>(a, dt, 0.0, epi, t.i0, 0, smem, r, nanflag);
   if (ISTEP == 2) {
     __syncthreads();
     block_reduce_store<TX>(r, nanflag, valid, sc, smem);
   }
 }
 
-// Step end of every running member: the persistent loop's dt, time, step
-// count and status update; a member that stops writes its EnsOut.
-__global__ void k_lock_end(EnsArgs e, DevScalars* scs) {
-  const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (m >= e.n_members) return;
-  DevScalars* sc = scs + m;
-  if (!sc->active) return;
-  const EnsMember M = e.mem[m];
-  const RoundOut red = {as_f64(sc->red[RED_DT]), as_f64(sc->red[RED_RHOAC]), 0.0, 0.0};
-  const bool fl = sc->red[RED_NAN] != 0ull;
-  sc->red[RED_DT] = as_u64(1.0e300);
-  sc->red[RED_RHOAC] = 0ull;
-  sc->red[RED_GMAX] = 0ull;
-  sc->red[RED_IONPOT] = 0ull;
-  sc->red[RED_NAN] = 0ull;
-  const double dt = sc->dt;
-  const double dt_new = ens_dt(red, e.ph.safety);
-  const double time = sc->time + dt;
-  const long long nt = sc->nt + 1;
-  int status = sc->status;
-  if (dt <= 1.0e-12) status |= 1;
-  if (fl) status |= 2;
-  sc->time = time;
-  sc->nt = nt;
-  sc->status = status;
-  if (time < M.tend && nt < M.nmax && status == 0) {
-    sc->dt = dt_new;
-    sc->last = ((time + dt_new >= M.tend) || (nt + 1 >= M.nmax) || (dt_new <= 1.0e-12)) ? 1 : 0;
-  } else {
-    sc->active = 0;
-    e.out[m] = EnsOut{time, dt, nt, status, 0};
-  }
+// Step end of the listed members (those of the chunk's table) after step k,
+// p = k & 1: from buffer p into buffer p ^ 1; a member that stops writes its
+// EnsOut.
+__global__ void k_lock_end(EnsArgs e, DevScalars* scs, const int* members, int count, int p) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= count) return;
+  const int m = members[q];
+  bool stops;
+  const DevScalars nw = lock_step_end(scs[(long long)p * e.n_members + m], e.mem[m], e.ph.safety, &stops);
+  scs[(long long)(p ^ 1) * e.n_members + m] = nw;
+  if (stops) e.out[m] = EnsOut{nw.time, nw.dt, nw.nt, nw.status, 0};
 }
 }  // namespace pip

@@ -932,6 +932,8 @@ struct pip_ens {
   LockTile* tiles = nullptr;        // lockstep: (member, tile) table of the running members
   LockTile* h_tiles = nullptr;      // pinned staging of the table
   DevScalars* h_scs = nullptr;      // pinned copy of the scalars (which members still run)
+  int* list = nullptr;              // lockstep: members of the current table
+  int* h_list = nullptr;            // pinned staging of the list
   int max_tiles = 0;
 };
 
@@ -970,8 +972,11 @@ static int ens_launch(pip_ens* e) {
 }
 
 // Lockstep run: start kernel, then chunks of steps over the tile table of
-// the members still running; the table is rebuilt between chunks from the
-// members' active flags (one 64-step chunk per host round trip).
+// the members still running, each step two stage launches (stage 1 applies
+// the previous step's end), each chunk closed by k_lock_end over its
+// members; the table is rebuilt between chunks from the members' active
+// flags.  A member once stopped is never listed again (its other parity
+// buffer may still hold its last running state).
 template <int TX>
 static int ens_lockstep_t(pip_ens* e) {
   constexpr int TC = TX - 1;
@@ -1002,28 +1007,38 @@ static int ens_lockstep_t(pip_ens* e) {
   hipLaunchKernelGGL(k_lock_start<TX>, dim3((unsigned)e->n), dim3(TX), lds_start, e->stream, a, e->scs);
   HIPCHK(hipGetLastError());
   std::vector<char> run(e->n, 1);  // members that may still run (all, until the start kernel has said)
-  const unsigned gend = (unsigned)((e->n + 255) / 256);
-  const int chunk = 64;
+  constexpr int chunk = 64;        // even: a chunk ends on the parity it started with
+  long long k = 0;                 // global step index (parity of the scalar buffers)
   for (;;) {
-    int nt = 0;
+    int nt = 0, nm = 0;
     for (int m = 0; m < e->n; m++) {
       if (!run[m]) continue;
+      e->h_list[nm++] = m;
       for (int i0 = 0; i0 < e->hmem[m].nx; i0 += TC) e->h_tiles[nt++] = LockTile{m, i0};
     }
     if (nt == 0) break;
     if (nt > e->max_tiles) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
     HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
-    for (int k = 0; k < chunk; k++) {
+    HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)nm, hipMemcpyHostToDevice, e->stream));
+    for (int s = 0; s < chunk; s++) {
+      const int p = (int)((k + s) & 1);
       hipLaunchKernelGGL((k_lock_stage<TX, 1>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
-                         e->tiles);
+                         e->tiles, p, s > 0 ? 1 : 0);
       hipLaunchKernelGGL((k_lock_stage<TX, 2>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
-                         e->tiles);
-      hipLaunchKernelGGL(k_lock_end, dim3(gend), dim3(256), 0, e->stream, a, e->scs);
+                         e->tiles, p, 0);
     }
+    const int pl = (int)((k + chunk - 1) & 1);
+    hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, e->stream, a, e->scs, e->list,
+                       nm, pl);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(e->h_scs, e->scs, sizeof(DevScalars) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(e->h_scs, e->scs + (size_t)(pl ^ 1) * e->n, sizeof(DevScalars) * (size_t)e->n,
+                          hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    for (int m = 0; m < e->n; m++) run[m] = (char)(e->h_scs[m].active != 0);
+    k += chunk;
+    for (int q = 0; q < nm; q++) {
+      const int m = e->h_list[q];
+      run[m] = (char)(e->h_scs[m].active != 0);
+    }
   }
   return PIP_OK;
 }
@@ -1123,7 +1138,10 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   if ((er = hipMalloc(&e->order, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc order");
   if ((er = hipMalloc(&e->counter, sizeof(unsigned) * 4)) != hipSuccess) return bail("hipMalloc counter");
   for (int m = 0; m < n; m++) e->max_tiles += (members[m].nx + 62) / 63;  // tiles of the narrowest width
-  if ((er = hipMalloc(&e->scs, sizeof(DevScalars) * n)) != hipSuccess) return bail("hipMalloc scalars");
+  if ((er = hipMalloc(&e->scs, 2 * sizeof(DevScalars) * n)) != hipSuccess) return bail("hipMalloc scalars");
+  if ((er = hipMalloc(&e->list, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc list");
+  if ((er = hipHostMalloc((void**)&e->h_list, sizeof(int) * n, hipHostMallocDefault)) != hipSuccess)
+    return bail("hipHostMalloc list");
   if ((er = hipMalloc(&e->tiles, sizeof(LockTile) * e->max_tiles)) != hipSuccess) return bail("hipMalloc tiles");
   if ((er = hipHostMalloc((void**)&e->h_tiles, sizeof(LockTile) * e->max_tiles, hipHostMallocDefault)) != hipSuccess)
     return bail("hipHostMalloc tiles");
@@ -1152,7 +1170,9 @@ int pip_ens_destroy(pip_ens* e) {
   (void)hipFree(e->counter);
   (void)hipFree(e->scs);
   (void)hipFree(e->tiles);
+  (void)hipFree(e->list);
   if (e->h_tiles) (void)hipHostFree(e->h_tiles);
+  if (e->h_list) (void)hipHostFree(e->h_list);
   if (e->h_scs) (void)hipHostFree(e->h_scs);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;

@@ -63,16 +63,19 @@ def test_ensemble_rerun_after_reset_is_identical():
         assert np.array_equal(a[k][0], b[0]) and np.array_equal(a[k][1], b[1])
 
 
-def test_lockstep_equals_persistent_at_the_step_cap():
+@pytest.mark.parametrize("cap", [150, 128, 64])
+def test_lockstep_equals_persistent_at_the_step_cap(cap):
     """Lockstep and persistent launches give the same states (ghosts
     included), times, step counts and statuses when the members stop at the
-    step cap inside a 64-step chunk, and again after a reset."""
-    print("case ensemble lockstep vs persistent, 12 members, cap 150 steps", flush=True)
+    step cap inside a 64-step chunk (150: the next step's stage 1 applies the
+    step end) or at a chunk's last step (128, 64: the chunk's closing step
+    end), and again after a reset."""
+    print("case ensemble lockstep vs persistent, 12 members, cap", cap, flush=True)
     mem = _members(12, nx_cap=1500)
-    ens = Ensemble(mem, tend=0.1, max_steps=150, tile=128)
+    ens = Ensemble(mem, tend=0.1, max_steps=cap, tile=128)
     ens.run(graph=False)
     ref = [(ens.result(k), ens.state(k)) for k in range(12)]
-    assert all(r["nt"] == 150 for r, _ in ref)
+    assert all(r["nt"] == cap for r, _ in ref)
     ens.set_mode("lockstep")
     for _ in range(2):
         ens.reset()
