@@ -118,6 +118,25 @@ __device__ __forceinline__ RoundOut cfl_round(double* um, double* uh, double ac,
   return r;
 }
 
+// The state rewrites of one round alone, for a round whose dt terms are not
+// used (the first of the two rounds of a fused step): set_IR's get_Pr
+// rewrites, cfl's cq2pv_mhd / cq2pv_hd and, for IR type 0, cfl_pip_ir's
+// get_Pr_MHD, in the order cfl_round applies them.
+template <int NV, bool IR>
+__device__ __forceinline__ void cfl_rewrite(double* um, double* uh, const StageArgs& a) {
+  if (IR) {
+    double gr, gi;
+    set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
+  }
+  double pm[8], ph[5];
+  cq2pv_mhd(um, pm, a.ph);
+  cq2pv_hd(uh, ph, a.ph);
+  if (IR && a.ph.flag_ir_type == 0) {
+    double pr, te;
+    getpr_mhd(um, pr, te, a.ph);
+  }
+}
+
 // Workgroup min/max reduction of the dt terms, then one atomic per slot.
 template <int NT>
 __device__ __forceinline__ void block_reduce_store(RoundOut r, int nanflag, bool valid, DevScalars* sc,
@@ -387,8 +406,8 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
     if (epi >= 1) {
       double* um = un;
       double* uh = un + NV;
+      if (epi >= 2) cfl_rewrite<NV, IR>(um, uh, a);  // main.f90:44-47, its dt unused
       r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
-      if (epi >= 2) r = cfl_round<NV, IR>(um, uh, acc, ion_pot, a);
 #pragma unroll
       for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
     }
@@ -640,8 +659,8 @@ __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
       un[8] = un[8] * fac;
     }
     if (epi >= 1) {
+      if (epi >= 2) cfl_rewrite<NV, false>(un, un + NV, a);  // main.f90:44-47, its dt unused
       r = cfl_round<NV, false>(un, un + NV, acc, ion_pot, a);
-      if (epi >= 2) r = cfl_round<NV, false>(un, un + NV, acc, ion_pot, a);
 #pragma unroll
       for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
     }
