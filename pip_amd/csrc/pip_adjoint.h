@@ -12,10 +12,10 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
   adj_cell_item(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Stencil kernels cover their index space with 64x4 tiles, one per
-// 256-thread workgroup.  Workgroups are dealt round-robin to the 8 XCDs; the
-// tile order gives each XCD a contiguous range of tiles, so that the rows a
-// tile shares with its neighbours (y stencil) meet in one L2.
+// The gather covers the interior cells with 64x4 tiles, one per 256-thread
+// workgroup.  Workgroups are dealt round-robin to the 8 XCDs; the tile order
+// gives each XCD a contiguous range of tiles (and of face blocks, below), so
+// that the rows a tile shares with its neighbours (y stencil) meet in one L2.
 constexpr int ADJ_TW = 64, ADJ_TH = 4;
 
 __device__ __forceinline__ int xcd_order(int bid, int nb) {
@@ -35,18 +35,25 @@ __device__ __forceinline__ bool adj_tile_item(int t, int w, int h, int& x, int& 
   return x < w && y < h;
 }
 
-// Faces of both directions of one fluid in one launch: the first
-// adj_tiles(nx+1, ny) tiles are x faces, the rest y faces.
+// Faces of both directions of one fluid in one launch, 256 consecutive
+// faces (row-major) per workgroup: the first adj_face_blocks(nx+1, ny)
+// workgroups take the x faces, the rest the y faces.  Rows of nx+1 x faces
+// are not a multiple of 64, so rectangular tiles would leave waves with a
+// single live lane running the whole VJP.
+__host__ __device__ inline int adj_face_blocks(int w, int h) {
+  return (int)(((long long)w * h + 255) / 256);
+}
+
 template <int FL>
 __global__ void __launch_bounds__(256, FL ? 2 : 1) k_adj_faces(AdjArgs a) {
-  const int ntx = adj_tiles(a.nx + 1, a.ny);
-  int t = xcd_order(blockIdx.x, gridDim.x);
-  const int dir = (t < ntx) ? 0 : 1;
-  if (dir) t -= ntx;
+  const int nbx = adj_face_blocks(a.nx + 1, a.ny);
+  int b = xcd_order(blockIdx.x, gridDim.x);
+  const int dir = (b < nbx) ? 0 : 1;
+  if (dir) b -= nbx;
   const int w = dir ? a.nx : a.nx + 1, h = dir ? a.ny + 1 : a.ny;
-  int x, y;
-  if (!adj_tile_item(t, w, h, x, y)) return;
-  adj_face_item<FL>(a, dir, (long long)y * w + x);
+  const long long t = (long long)b * 256 + threadIdx.x;
+  if (t >= (long long)w * h) return;
+  adj_face_item<FL>(a, dir, t);
 }
 
 // Interior tiles; in stage 1 (lam_in then receives the step's result) the

@@ -785,7 +785,7 @@ static int adj_stage(pip_ctx* c, const AdjArgs& a) {
   // the primitives, when the replay did not save them, and (then) the stage-1
   // ghost zeroing of lam_in; otherwise the gather's extra blocks zero them
   if (a.make_V) hipLaunchKernelGGL(k_adj_cell, cell_grid(c), dim3(256), 0, c->stream, a);
-  const dim3 gf((unsigned)(adj_tiles(c->nx + 1, c->ny) + adj_tiles(c->nx, c->ny + 1)));
+  const dim3 gf((unsigned)(adj_face_blocks(c->nx + 1, c->ny) + adj_face_blocks(c->nx, c->ny + 1)));
   hipLaunchKernelGGL(k_adj_faces<0>, gf, dim3(256), 0, c->stream, a);
   hipLaunchKernelGGL(k_adj_faces<1>, gf, dim3(256), 0, c->stream, a);
   const int ghost = (a.istep == 1 && !a.make_V) ? adj_ghost_blocks(c->cfg.ix, c->cfg.jx) : 0;
