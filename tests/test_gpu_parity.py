@@ -68,6 +68,20 @@ def test_brio_wu_1d_to_tend():
         assert jo == jg
 
 
+@pytest.mark.parametrize("rho_r,p_r", [(1.0e-4, 1.0e-3), (2.0e-8, 5.0e-9)])
+def test_strong_blast_and_near_vacuum_1d(rho_r, p_r):
+    """Wide dynamic range for the solvers' quotient and square-root
+    sequences: a density jump of 1e4 and pressure jump of 1e6, and a
+    near-vacuum right state at the density and pressure floors (1e-8), for
+    300 steps, bitwise against the oracle."""
+    say("case blast rho_r", rho_r, "p_r", p_r, "nx=512 steps=300")
+    cfg, uh, um, ac = problems.brio_wu(512, col=1.0, p_left=1.0e3, p_right=p_r, rho_left=1.0, rho_right=rho_r)
+    o, s, n, _ = run_pair(cfg, uh, um, ac, nsteps=300)
+    assert n == 300
+    compare(o, s, cfg, 0.0)
+    assert s.scalars()["dt"] == o.scalars()["dt"]
+
+
 def test_orszag_tang_small_100_steps():
     say("case orszag_tang n=64 steps=100")
     cfg, uh, um, ac = problems.orszag_tang(64)
