@@ -1063,6 +1063,50 @@ static int ens_enqueue(pip_ens* e) {
   return fail(PIP_ERR_ARG, "ensemble tile must be 64, 128, 256 or 512");
 }
 
+// The ensemble run between events a and b on the ensemble's stream (the
+// graph, when asked for, is captured on first use and kept); *ms = the
+// device time between them.
+static int ens_run_between(pip_ens* e, int use_graph, hipEvent_t a, hipEvent_t b, float* ms) {
+  if (e->mode == 1) {  // host-driven chunks: no graph
+    HIPCHK(hipEventRecord(a, e->stream));
+    TRY(ens_lockstep(e));
+    HIPCHK(hipEventRecord(b, e->stream));
+  } else if (use_graph) {
+    if (!e->gexec || e->graph_tx != e->tx) {
+      if (e->gexec) {
+        HIPCHK(hipGraphExecDestroy(e->gexec));
+        e->gexec = nullptr;
+      }
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = ens_enqueue(e);
+      const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (rc != PIP_OK) {
+        if (ce == hipSuccess) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ce != hipSuccess) return fail(PIP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        e->gexec = nullptr;
+        return fail(PIP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      }
+      e->graph_tx = e->tx;
+    }
+    HIPCHK(hipEventRecord(a, e->stream));
+    HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    HIPCHK(hipEventRecord(b, e->stream));
+  } else {
+    HIPCHK(hipEventRecord(a, e->stream));
+    TRY(ens_enqueue(e));
+    HIPCHK(hipEventRecord(b, e->stream));
+  }
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  return PIP_OK;
+}
+
 extern "C" {
 
 int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pip_ens** out) {
@@ -1222,46 +1266,18 @@ int pip_ens_reset(pip_ens* e) {
 
 int pip_ens_run(pip_ens* e, int use_graph, double* ms) {
   if (!e) return fail(PIP_ERR_ARG, "null ensemble");
-  hipEvent_t a, b;
+  hipEvent_t a = nullptr, b = nullptr;
   HIPCHK(hipEventCreate(&a));
-  HIPCHK(hipEventCreate(&b));
-  if (e->mode == 1) {  // host-driven chunks: no graph
-    HIPCHK(hipEventRecord(a, e->stream));
-    int rc = ens_lockstep(e);
-    if (rc != PIP_OK) return rc;
-    HIPCHK(hipEventRecord(b, e->stream));
-  } else if (use_graph) {
-    if (!e->gexec || e->graph_tx != e->tx) {
-      if (e->gexec) {
-        HIPCHK(hipGraphExecDestroy(e->gexec));
-        e->gexec = nullptr;
-      }
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      int rc = ens_enqueue(e);
-      hipError_t ce = hipStreamEndCapture(e->stream, &g);
-      if (rc != PIP_OK) return rc;
-      if (ce != hipSuccess) return fail(PIP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
-      HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
-      e->graph_tx = e->tx;
-    }
-    HIPCHK(hipEventRecord(a, e->stream));
-    HIPCHK(hipGraphLaunch(e->gexec, e->stream));
-    HIPCHK(hipEventRecord(b, e->stream));
-  } else {
-    HIPCHK(hipEventRecord(a, e->stream));
-    int rc = ens_enqueue(e);
-    if (rc != PIP_OK) return rc;
-    HIPCHK(hipEventRecord(b, e->stream));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(PIP_ERR_HIP, "pip_ens_run: hipEventCreate");
   }
-  HIPCHK(hipEventSynchronize(b));
   float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, a, b));
+  const int rc = ens_run_between(e, use_graph, a, b, &t);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  if (ms) *ms = t;
-  return PIP_OK;
+  if (rc == PIP_OK && ms) *ms = t;
+  return rc;
 }
 
 int pip_ens_result(pip_ens* e, int m, double* time, long* nt, int* status) {
