@@ -1014,10 +1014,12 @@ static int ens_lockstep_t(pip_ens* e) {
     for (int m = 0; m < e->n; m++) {
       if (!run[m]) continue;
       e->h_list[nm++] = m;
-      for (int i0 = 0; i0 < e->hmem[m].nx; i0 += TC) e->h_tiles[nt++] = LockTile{m, i0};
+      for (int i0 = 0; i0 < e->hmem[m].nx; i0 += TC) {
+        if (nt >= e->max_tiles) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
+        e->h_tiles[nt++] = LockTile{m, i0};
+      }
     }
     if (nt == 0) break;
-    if (nt > e->max_tiles) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
     HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)nm, hipMemcpyHostToDevice, e->stream));
     for (int s = 0; s < chunk; s++) {
