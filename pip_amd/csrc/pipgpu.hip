@@ -301,7 +301,8 @@ extern "C" {
 const char* pip_last_error(void) { return g_err.c_str(); }
 
 int pip_check_arith(const double* n, const double* d, double* q, double* r, long long count) {
-  if (count < 0 || (count > 0 && !(n && d && q && r))) return fail(PIP_ERR_ARG, "pip_check_arith: bad arguments");
+  if (count < 0 || count > (1LL << 31) || (count > 0 && !(n && d && q && r)))
+    return fail(PIP_ERR_ARG, "pip_check_arith: bad arguments (0 <= count <= 2^31, non-null arrays)");
   if (count == 0) return PIP_OK;
   const size_t b = (size_t)count * sizeof(double);
   double* buf = nullptr;
