@@ -61,3 +61,8 @@ def test_null_arguments_are_errors():
     assert L.pip_create(None, None) == -1
     assert L.pip_destroy(None) == 0
     assert L.pip_sync(None) == -1
+    assert L.pip_ens_set_mode(None, 1) == -1
+    # argument checks come before any device call
+    assert L.pip_check_arith(None, None, None, None, 4) == -1
+    assert L.pip_check_arith(None, None, None, None, -1) == -1
+    assert L.pip_check_arith(None, None, None, None, 0) == 0
