@@ -164,11 +164,17 @@ PIP_HD unsigned long long* adj_codes(const AdjArgs& a, int dir, int fl) {
   return reinterpret_cast<unsigned long long*>(a.P + (4LL * ADJ_NVT + 2 * dir + fl) * a.N);
 }
 
+// The adjoints of a face's left and right states of one variable, side by
+// side (one 16-byte access).
+struct alignas(16) FacePair {
+  double l, r;
+};
+
 // One thread per face and fluid (FL 0: plasma, HLLD with GLM; FL 1:
 // neutrals, HLLC); R = the cell on the + side of the face.
 // dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
-// Writes P[(dir*2 + side) * NVT + n][R]: the adjoints of the left (side 0)
-// and right (side 1) face states of variable n, density floor applied.
+// Writes the pair plane (dir * NVT + n) at R: the adjoints of the left and
+// right face states of variable n, density floor applied.
 template <int FL>
 PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
@@ -225,12 +231,11 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
   else hllc_flux_vjp(vl, vr, a.ph.gm, lamF, lvl, lvr);
   if (flL) lvl[0] = 0.0;
   if (flR) lvr[0] = 0.0;
-  double* P = a.P + (long long)dir * 2 * NVT * N;
+  FacePair* P = reinterpret_cast<FacePair*>(a.P) + (long long)dir * NVT * N;
 #pragma unroll
   for (int n = 0; n < NF; n++) {
     const int s = adj_perm(OFF + n, dir);
-    P[s * N + oR] = lvl[n];
-    P[(NVT + s) * N + oR] = lvr[n];
+    P[s * N + oR] = FacePair{lvl[n], lvr[n]};
   }
 }
 
@@ -256,19 +261,19 @@ PIP_HD void adj_gather_fluid(const AdjArgs& a, int dir, int fl, int n0, long lon
                              unsigned mask, double* lv) {
   constexpr int NVT = ADJ_NVT;
   const long long N = a.N;
-  const double* PL = a.P + (long long)dir * 2 * NVT * N;
-  const double* PR = PL + (long long)NVT * N;
+  const FacePair* P = reinterpret_cast<const FacePair*>(a.P) + (long long)dir * NVT * N;
   const unsigned long long* C = adj_codes(a, dir, fl);
   const unsigned long long k2 = C[o2], k1 = C[o1], k0 = C[o0], km = C[om];
 #pragma unroll
   for (int q = 0; q < NF; q++) {
     const int n = n0 + q, sh = 4 * q;
-    const double acc = v2 * (PL[n * N + o2] * wl((unsigned)(k2 >> sh) & 3u, 0)) +
-                       v1 * (PL[n * N + o1] * wl((unsigned)(k1 >> sh) & 3u, 1) +
-                             PR[n * N + o1] * wr((unsigned)(k1 >> (sh + 2)) & 3u, 0)) +
-                       v0 * (PL[n * N + o0] * wl((unsigned)(k0 >> sh) & 3u, 2) +
-                             PR[n * N + o0] * wr((unsigned)(k0 >> (sh + 2)) & 3u, 1)) +
-                       vm * (PR[n * N + om] * wr((unsigned)(km >> (sh + 2)) & 3u, 2));
+    const FacePair p2 = P[n * N + o2], p1 = P[n * N + o1], p0 = P[n * N + o0], pm = P[n * N + om];
+    const double acc = v2 * (p2.l * wl((unsigned)(k2 >> sh) & 3u, 0)) +
+                       v1 * (p1.l * wl((unsigned)(k1 >> sh) & 3u, 1) +
+                             p1.r * wr((unsigned)(k1 >> (sh + 2)) & 3u, 0)) +
+                       v0 * (p0.l * wl((unsigned)(k0 >> sh) & 3u, 2) +
+                             p0.r * wr((unsigned)(k0 >> (sh + 2)) & 3u, 1)) +
+                       vm * (pm.r * wr((unsigned)(km >> (sh + 2)) & 3u, 2));
     lv[n] += ((mask >> n) & 1u) ? -acc : acc;
   }
 }

@@ -69,7 +69,7 @@ struct pip_ctx {
   int device = 0;
   // adjoint work buffers (allocated on the first pip_adj_step)
   double* adjV = nullptr;   // nvt planes
-  double* adjP = nullptr;   // 4 * nvt planes (2 dirs x 2 face sides) + 4 planes of routing codes
+  double* adjP = nullptr;   // 2 dirs x nvt planes of (left, right) face-state pairs + 4 planes of routing codes
   double* lamB = nullptr;   // nvt planes
 };
 
