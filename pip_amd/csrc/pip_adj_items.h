@@ -160,8 +160,13 @@ PIP_HD int adj_perm(int n, int dir) {
 // Packed routing codes of the faces of one direction and fluid: bits
 // 4k..4k+1 the left-state branch, 4k+2..4k+3 the right-state branch of the
 // fluid's k-th variable (recon_code), at the face's + cell.
-PIP_HD unsigned long long* adj_codes(const AdjArgs& a, int dir, int fl) {
-  return reinterpret_cast<unsigned long long*>(a.P + (4LL * ADJ_NVT + 2 * dir + fl) * a.N);
+// The codes of both fluids at a face sit side by side (one 16-byte
+// access): code word (dir, cell, fluid) at ((dir * N + cell) * 2 + fluid).
+struct alignas(16) CodePair {
+  unsigned long long c[2];
+};
+PIP_HD CodePair* adj_code_pairs(const AdjArgs& a, int dir) {
+  return reinterpret_cast<CodePair*>(a.P + 4LL * ADJ_NVT * a.N) + (long long)dir * a.N;
 }
 
 // The adjoints of a face's left and right states of one variable, side by
@@ -218,7 +223,7 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
       const unsigned cr = recon_code(p[-st], p[0], p[st]);
       codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
     }
-    adj_codes(a, dir, FL)[oR] = codes;
+    adj_code_pairs(a, dir)[oR].c[FL] = codes;
   }
   // density floor of get_Side (HLL_rot.f90:60-61)
   const bool flL = !(vl[0] > a.ph.ro_lim), flR = !(vr[0] > a.ph.ro_lim);
@@ -254,16 +259,16 @@ PIP_HD double wr(unsigned code, int q) {
 
 // Contributions of the faces of one direction and fluid to the cell:
 // variables n0..n0+NF-1, faces with + cells c+2, c+1, c, c-1 (offsets
-// o2, o1, o0, om, clamped to valid faces) scaled by v2..vm (1 valid, 0 not).
+// o2, o1, o0, om, clamped to valid faces) scaled by v2..vm (1 valid, 0 not),
+// with the fluid's branch codes k2..km at those faces.
 template <int NF>
-PIP_HD void adj_gather_fluid(const AdjArgs& a, int dir, int fl, int n0, long long o2, long long o1,
-                             long long o0, long long om, double v2, double v1, double v0, double vm,
-                             unsigned mask, double* lv) {
+PIP_HD void adj_gather_fluid(const AdjArgs& a, int dir, int n0, long long o2, long long o1, long long o0,
+                             long long om, double v2, double v1, double v0, double vm,
+                             unsigned long long k2, unsigned long long k1, unsigned long long k0,
+                             unsigned long long km, unsigned mask, double* lv) {
   constexpr int NVT = ADJ_NVT;
   const long long N = a.N;
   const FacePair* P = reinterpret_cast<const FacePair*>(a.P) + (long long)dir * NVT * N;
-  const unsigned long long* C = adj_codes(a, dir, fl);
-  const unsigned long long k2 = C[o2], k1 = C[o1], k0 = C[o0], km = C[om];
 #pragma unroll
   for (int q = 0; q < NF; q++) {
     const int n = n0 + q, sh = 4 * q;
@@ -297,8 +302,10 @@ PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double*
                     o0 = o + (long long)(clampf(c) - c) * st, om = o + (long long)(clampf(c - 1) - c) * st;
     const double v2 = (c + 2 >= 2 && c + 2 <= hi) ? 1.0 : 0.0, v1 = (c + 1 >= 2 && c + 1 <= hi) ? 1.0 : 0.0,
                  v0 = (c >= 2 && c <= hi) ? 1.0 : 0.0, vm = (c - 1 >= 2 && c - 1 <= hi) ? 1.0 : 0.0;
-    adj_gather_fluid<NV>(a, dir, 0, 0, o2, o1, o0, om, v2, v1, v0, vm, mask, lv);
-    adj_gather_fluid<5>(a, dir, 1, NV, o2, o1, o0, om, v2, v1, v0, vm, mask, lv);
+    const CodePair* C = adj_code_pairs(a, dir);
+    const CodePair k2 = C[o2], k1 = C[o1], k0 = C[o0], km = C[om];
+    adj_gather_fluid<NV>(a, dir, 0, o2, o1, o0, om, v2, v1, v0, vm, k2.c[0], k1.c[0], k0.c[0], km.c[0], mask, lv);
+    adj_gather_fluid<5>(a, dir, NV, o2, o1, o0, om, v2, v1, v0, vm, k2.c[1], k1.c[1], k0.c[1], km.c[1], mask, lv);
   }
 }
 
