@@ -85,3 +85,24 @@ def test_lockstep_equals_persistent_at_the_step_cap(cap):
             assert ens.result(k) == r, k
             eh, em = ens.state(k)
             assert np.array_equal(eh, uh) and np.array_equal(em, um), k
+
+
+def test_lockstep_and_persistent_agree_on_48_members():
+    """48 members of uneven size (chunks drop members at different steps):
+    lockstep at width 128 against the persistent launch at width 256, every
+    state (ghosts included), time, step count and status equal."""
+    print("case ensemble 48 members lockstep 128 vs persistent 256, t_end 0.02", flush=True)
+    mem = _members(48, nx_cap=800)
+    ens = Ensemble(mem, tend=0.02, max_steps=MAXS, tile=256)
+    ens.run(graph=True)
+    ref = [(ens.result(k), ens.state(k)) for k in range(48)]
+    assert len({r["nt"] for r, _ in ref}) > 10  # members stop at many different steps
+    ens.set_mode("lockstep")
+    ens.set_tile(128)
+    ens.reset()
+    ens.run()
+    for k in range(48):
+        r, (uh, um) = ref[k]
+        assert ens.result(k) == r, k
+        eh, em = ens.state(k)
+        assert np.array_equal(eh, uh) and np.array_equal(em, um), k
