@@ -1,22 +1,31 @@
 """Benchmark of the fused two-fluid step on one MI355X.
 
-Workload: the 2-D two-fluid Orszag-Tang problem on a 1024^2 periodic box in
-FP64 (xi_n=0.9, alpha_c=1e3, Dedner psi, SSP-RK2, CFL 0.3), 500 steps: one
-"step" is one full pass of the time loop (two fused stage kernels + the dt
-finalisation) over the whole grid, with the state resident in HBM.
+Headline workload (default): the 2-D two-fluid Kelvin-Helmholtz problem with
+temperature-dependent ionisation/recombination (set_IR case 3, ion_pot /
+arb_heat energy terms) at 2048^2 in FP64: x periodic, y reflecting, xi_n=0.8,
+alpha_c=1e2, Dedner psi, SSP-RK2, CFL 0.3, 1000 steps.  One "step" is one
+full pass of the time loop (two fused stage kernels + the dt finalisation)
+over the whole grid, with the state resident in HBM.
 
-Prints one JSON line (see the repository documentation for the fields):
-value = interior cells x steps / wall time of the timed steps;
-roofline = algorithmic HBM bytes of the dominant kernel (the stage-2 kernel:
-3*nv*8 B per cell) / its average duration measured with hipEvents on the
-library's stream over the same timed steps; cpu_baseline = the CPU oracle
-(single core) on a bounded sample of the same workload.
-Two more objects compare the training-system patterns with plain runs:
-batched = 256 uneven 1-D shock tubes in one persistent hipGraph launch
-against the same 256 runs as sequential plain pip_advance calls (results
-compared bit for bit); adjoint = gradient (checkpointed forward + reverse
-sweep) of the 512^2 Alfven-wave damping run (200 steps, checkpoints every
-20) against the plain forward run.
+Prints one JSON line:
+  value      interior cells x steps / wall time of the timed steps;
+  roofline   the stage kernel k_stage (both SSP-RK stages are launches of
+             it): algorithmic HBM bytes of a step (stage 1 reads U^n and
+             writes U^(1); stage 2 reads U^(1), U^n and writes U^(n+1);
+             every stage reads the arb_heat field) over the summed duration
+             of its two launches, both measured with hipEvents on the
+             library's stream over the timed steps; traffic = measured HBM
+             bytes per launch from the committed rocprofv3 summary;
+  cpu_baseline   the CPU oracle (the C restatement of the reference path) on
+             one host core, median of three timed steps of the same
+             workload; cpu_baseline_threads the same on all host cores
+             (OpenMP, up to 16 threads);
+  secondary  the Orszag-Tang 1024^2 run (the 2-D configuration without IR);
+  batched    256 uneven 1-D shock tubes as one graph-captured ensemble
+             against the same runs as sequential plain launches;
+  adjoint    gradient (checkpointed forward + reverse sweep) of the 512^2
+             Alfven-wave damping run (200 steps, checkpoints every 20)
+             against the plain forward run.
 """
 from __future__ import annotations
 
@@ -24,6 +33,7 @@ import argparse
 import json
 import os
 import re
+import statistics
 import sys
 import time
 
@@ -33,51 +43,90 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# committed rocprofv3 --pmc summaries of the stage-2 kernel: the one-pass
-# k_stage and the fluid-by-fluid k_stage2p
-PROFILE_PMC = {False: os.path.join(ROOT, "profiles", "stage2_orszag_tang_1024_pmc.md"),
-               True: os.path.join(ROOT, "profiles", "stage2p_orszag_tang_1024_pmc.md")}
+WORKLOADS = {
+    # name: (problem builder name, default n, config fields of the JSON line)
+    "kh": ("kelvin_helmholtz", 2048),
+    "ot": ("orszag_tang", 1024),
+}
+WL_NAME = {"kh": "kelvin_helmholtz_2d_ir", "ot": "orszag_tang_2d_two_fluid"}
 
 
-def two_pass_stage2(tile) -> bool:
-    return len(tile) > 3 and bool((int(tile[3]) >> 1) & 1)
+def pmc_profile_path(wl: str, n: int) -> str:
+    return os.path.join(ROOT, "profiles", f"k_stage_{WL_NAME[wl]}_{n}_pmc.md")
 
 
-def traffic_from_profile(tile):
-    """(HBM bytes per stage-2 launch, tile it was measured with) from the
-    committed summary of the stage-2 kernel the tile selects (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = PROFILE_PMC[two_pass_stage2(tile)]
+def make_problem(wl: str, n: int):
+    from pip_amd import problems
+    return getattr(problems, WORKLOADS[wl][0])(n)
+
+
+def workload_config(wl: str, n: int, cfg, tile):
+    c = {"workload": WL_NAME[wl], "nx": n, "ny": n, "nvar": cfg.nvar_m + cfg.nvar_h,
+         "flag_divb": cfg.flag_divb, "ssp_rk": cfg.t_order, "cfl": cfg.safety,
+         "alpha_c": cfg.col, "flag_bnd": list(cfg.flag_bnd)}
+    if wl == "kh":
+        c.update({"xi_n": 0.8, "flag_ir": cfg.flag_ir, "flag_ir_type": cfg.flag_ir_type,
+                  "shear": "vx = 0.5 tanh(y/0.05)", "bx": 0.1, "seed": 20241202})
+    else:
+        c.update({"xi_n": 0.9})
+    c.update({"tile": list(tile[:2]), "xcd_remap": int(tile[2]) if len(tile) > 2 else 0,
+              "two_pass_stages": int(tile[3]) if len(tile) > 3 else 0})
+    return c
+
+
+def traffic_from_profile(wl: str, n: int):
+    """(HBM bytes per k_stage launch averaged over both stages, per stage-1
+    and per stage-2 launch, tile measured) from the committed summary."""
+    path = pmc_profile_path(wl, n)
     if not os.path.exists(path):
-        return None, None
+        return None
     txt = open(path).read()
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+
+    def num(key):
+        m = re.search(key + r":\s*([0-9.eE+]+)", txt)
+        return float(m.group(1)) if m else None
     t = re.search(r"measured with tile:\s*(\S+)", txt)
-    return (float(m.group(1)) if m else None), (t.group(1) if t else None)
+    return {"mean": num("traffic_bytes_per_launch"), "stage1": num("stage1_bytes_per_launch"),
+            "stage2": num("stage2_bytes_per_launch"), "tile": t.group(1) if t else None}
+
+
+def _csv_rows(d: str, pattern: str):
+    import csv
+    import glob
+    out = []
+    for f in sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True)):
+        out += list(csv.DictReader(open(f)))
+    return out
 
 
 def _pmc_rows(d: str, counter: str):
     """(kernel name, value) per dispatch of one counter from the rocprofv3
     --pmc CSVs under d, in dispatch order."""
-    import csv
-    import glob
-    rows = []
-    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-        for r in csv.DictReader(open(f)):
-            if r.get("Counter_Name") == counter:
-                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows = [(int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"]))
+            for r in _csv_rows(d, "*counter_collection.csv") if r.get("Counter_Name") == counter]
     rows.sort()
     return [(k, v) for _, k, v in rows]
 
 
-def write_pmc_summary(d: str, cmds: dict, tile: str):
-    """HBM bytes per stage-2 k_stage launch from four rocprofv3 --pmc runs
-    (d/fetch, d/write: the main workload; d/copy_fetch, d/copy_write: the
-    1 GiB copy).  FETCH_SIZE / WRITE_SIZE are in KiB per dispatch; each is
-    scaled by known bytes / counted bytes of the copy kernel."""
-    def stage2(counter, sub):
+def resolved_tile(wl: str, n: int, tile: str) -> str:
+    """The tile string a run with --tile `tile` uses (auto: the autotuner's
+    cached choice for the workload, if any)."""
+    if tile != "auto":
+        return tile
+    from pip_amd import autotune
+    cfg = make_problem(wl, min(n, 64))[0].replace(nx=n, ny=n)
+    hit = autotune._load().get(autotune.stage_key(cfg))
+    return "x".join(str(v) for v in hit["tile"]) if hit else "auto (no cached choice)"
+
+
+def write_pmc_summary(d: str, cmds: dict, wl: str, n: int, tile: str):
+    """HBM bytes per k_stage launch from four rocprofv3 --pmc runs (d/fetch,
+    d/write: the workload; d/copy_fetch, d/copy_write: the 1 GiB copy).
+    FETCH_SIZE / WRITE_SIZE are in KiB per dispatch; each is scaled by known
+    bytes / counted bytes of the copy kernel."""
+    def stages(counter, sub):
         v = [val for k, val in _pmc_rows(os.path.join(d, sub), counter) if "k_stage" in k]
-        return v[1::2], v[0::2]  # each step launches stage 1 then stage 2
+        return v[0::2], v[1::2]  # each step launches stage 1 then stage 2
 
     def copy_kib(counter, sub):
         v = [val for k, val in _pmc_rows(os.path.join(d, sub), counter) if "k_stage" not in k]
@@ -85,41 +134,173 @@ def write_pmc_summary(d: str, cmds: dict, tile: str):
     gib_kib = float(1 << 20)
     f_scale = gib_kib / copy_kib("FETCH_SIZE", "copy_fetch")
     w_scale = gib_kib / copy_kib("WRITE_SIZE", "copy_write")
-    f2, f1 = stage2("FETCH_SIZE", "fetch")
-    w2, w1 = stage2("WRITE_SIZE", "write")
+    f1, f2 = stages("FETCH_SIZE", "fetch")
+    w1, w2 = stages("WRITE_SIZE", "write")
     mean = lambda x: sum(x) / len(x)
-    fetch_b = mean(f2) * 1024 * f_scale
-    write_b = mean(w2) * 1024 * w_scale
-    ncell, nv = 1024 * 1024, 14
-    tt = tuple(int(v) for v in tile.split("x"))
-    kern = "k_stage2p (fluid by fluid)" if two_pass_stage2(tt) else "k_stage (one pass)"
-    lines = [f"# Stage-2 kernel {kern}, Orszag-Tang 1024^2 FP64: HBM traffic per launch", "",
-             f"measured with tile: {tile}", "",
-             "(each step launches stage 1 then stage 2)", "",
-             "Commands (each its own pass):", ""]
+    b1 = (mean(f1) * f_scale + mean(w1) * w_scale) * 1024
+    b2 = (mean(f2) * f_scale + mean(w2) * w_scale) * 1024
+    ncell = n * n
+    nv = 14  # both workloads: 9 plasma planes (with psi) + 5 neutral planes
+    extra = 8 if wl == "kh" else 0  # arb_heat read per stage
+    alg1 = (2 * nv * 8 + extra) * ncell
+    alg2 = (3 * nv * 8 + extra) * ncell
+    lines = [f"# k_stage, {WL_NAME[wl]} {n}^2 FP64: HBM traffic per launch", "",
+             f"measured with tile: {resolved_tile(wl, n, tile)}", "",
+             "(each step launches stage 1 then stage 2 of the same kernel)", "",
+             "Commands (each its own pass, from the repository root):", ""]
     lines += [f"    {c}" for c in cmds.values()]
     lines += ["", f"copy calibration: FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
               "(1 GiB read + 1 GiB written per copy)", "",
-              f"stage-2 launches: {len(f2)}; mean FETCH_SIZE {mean(f2):.1f} KiB, WRITE_SIZE {mean(w2):.1f} KiB",
-              f"stage-1 launches: {len(f1)}; mean FETCH_SIZE {mean(f1):.1f} KiB, WRITE_SIZE {mean(w1):.1f} KiB", "",
-              f"stage-2 HBM bytes per launch (scaled): read {fetch_b:.4e}, write {write_b:.4e}",
-              f"algorithmic bytes per launch: read {2 * nv * 8 * ncell:.4e}, write {nv * 8 * ncell:.4e}", "",
-              f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}", ""]
-    open(PROFILE_PMC[two_pass_stage2(tt)], "w").write("\n".join(lines))
+              f"stage-1 launches: {len(f1)}; mean FETCH_SIZE {mean(f1):.1f} KiB, WRITE_SIZE {mean(w1):.1f} KiB",
+              f"stage-2 launches: {len(f2)}; mean FETCH_SIZE {mean(f2):.1f} KiB, WRITE_SIZE {mean(w2):.1f} KiB", "",
+              f"stage1_bytes_per_launch: {b1:.6e}   (algorithmic {alg1:.6e}, ratio {b1 / alg1:.3f})",
+              f"stage2_bytes_per_launch: {b2:.6e}   (algorithmic {alg2:.6e}, ratio {b2 / alg2:.3f})",
+              f"traffic_bytes_per_launch: {(b1 + b2) / 2:.6e}   (mean of the two stages; algorithmic "
+              f"{(alg1 + alg2) / 2:.6e})", ""]
+    path = pmc_profile_path(wl, n)
+    open(path, "w").write("\n".join(lines))
     print("\n".join(lines))
 
 
-def cpu_baseline(n: int, steps: int):
-    from oracle.oracle import Oracle
-    from pip_amd import problems
-    cfg, uh, um, ac = problems.orszag_tang(n)
-    o = Oracle(cfg, uh, um, ac)
+SQ_PASSES = (("sq1", "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS"),
+             ("sq2", "SQ_WAVE_CYCLES SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_WAIT_INST_ANY"))
+
+
+def write_sq_summary(d: str, wl: str, n: int, tile: str, out: str):
+    """Instruction and wave-state counters of k_stage per launch (stage 1
+    and stage 2 separately) from the rocprofv3 --pmc passes d/sq1, d/sq2,
+    with the FP64 VALU ceiling they imply."""
+    main_args = f"--workload {wl} --n {n} --steps 10 --warmup 3 --no-cpu --no-extra --tile {tile}"
+    cmds = [f"cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc {c} --output-format csv "
+            f"-d /root/repo/gpurun_out/pmc/{sub} -o run -- python3 /root/repo/bench.py {main_args}"
+            for sub, c in SQ_PASSES]
+    vals = {}
+    for sub, cs in SQ_PASSES:
+        for c in cs.split():
+            v = [val for k, val in _pmc_rows(os.path.join(d, sub), c) if "k_stage" in k]
+            vals[c] = (sum(v[0::2]) / len(v[0::2]), sum(v[1::2]) / len(v[1::2]))
+    ncell = n * n
+    lanes = 256 * 4 * 16 * 2.4e9  # FP64 VALU lane-instructions per second (CU x SIMD x lanes x clock)
+    lines = [f"# k_stage, {WL_NAME[wl]} {n}^2 FP64: instruction and wave-state counters", "",
+             f"measured with tile: {resolved_tile(wl, n, tile)}", "",
+             "Commands (each its own pass, from the repository root):", ""]
+    lines += [f"    {c}" for c in cmds]
+    lines += ["", "Per launch (mean over the profiled launches; each step launches stage 1 then stage 2):", "",
+              "| counter | stage 1 | stage 2 |", "|---|---|---|"]
+    for c, (a, b) in vals.items():
+        lines.append(f"| {c} | {a:.4e} | {b:.4e} |")
+    w1, w2 = vals["SQ_WAVES"]
+    i1, i2 = vals["SQ_INSTS_VALU"]
+    lane_per_cell = (i1 + i2) * 64 / ncell
+    lines += ["", "Derived:", "",
+              f"- VALU instructions per wave: stage 1 {i1 / w1:.0f}, stage 2 {i2 / w2:.0f}",
+              f"- VALU lane-instructions per cell-update (both stages): {lane_per_cell:.0f}",
+              f"- FP64 VALU ceiling (256 CU x 4 SIMD x 16 lanes x 2.4 GHz = {lanes:.3e} lane-instr/s): "
+              f"{lanes / lane_per_cell:.3e} cell-updates/s",
+              "- share of a wave's cycles issuing VALU (SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES): "
+              f"stage 1 {vals['SQ_ACTIVE_INST_VALU'][0] / vals['SQ_WAVE_CYCLES'][0]:.2f}, "
+              f"stage 2 {vals['SQ_ACTIVE_INST_VALU'][1] / vals['SQ_WAVE_CYCLES'][1]:.2f}",
+              "- share waiting on a dependency (SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES): "
+              f"stage 1 {vals['SQ_WAIT_INST_ANY'][0] / vals['SQ_WAVE_CYCLES'][0]:.2f}, "
+              f"stage 2 {vals['SQ_WAIT_INST_ANY'][1] / vals['SQ_WAVE_CYCLES'][1]:.2f}",
+              "- lane utilisation (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)): "
+              f"stage 1 {vals['SQ_THREAD_CYCLES_VALU'][0] / (64 * vals['SQ_ACTIVE_INST_VALU'][0]):.3f}, "
+              f"stage 2 {vals['SQ_THREAD_CYCLES_VALU'][1] / (64 * vals['SQ_ACTIVE_INST_VALU'][1]):.3f}", ""]
+    open(out, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
+def write_stats_summary(d: str, cmd: str, title: str, out: str, note: str = ""):
+    """Kernel-trace summary (rocprofv3 --kernel-trace --stats, csv) as a
+    markdown table."""
+    rows = _csv_rows(d, "*kernel_stats.csv")
+    lines = [f"# {title}", "", "Command (from the repository root):", "", f"    {cmd}", ""]
+    if note:
+        lines += [note, ""]
+    lines += ["| kernel | calls | avg (us) | min (us) | max (us) | share |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                     f"{float(r['Percentage']):.2f} % |")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def cpu_baseline(wl: str, n: int, steps: int, threads: int):
+    """The oracle on `threads` host threads: `steps` consecutive steps of the
+    workload, each timed; the median step time gives the rate."""
+    from oracle import oracle as orc
+    prev = orc.set_threads(threads)
+    try:
+        cfg, uh, um, ac = make_problem(wl, n)
+        o = orc.Oracle(cfg, uh, um, ac)
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            o.step()
+            ts.append(time.perf_counter() - t0)
+        del o
+    finally:
+        orc.set_threads(prev)
+    med = statistics.median(ts)
+    return {"value": cfg.ncells / med, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference Fortran), {WL_NAME[wl]} {n}x{n}, "
+                      f"{steps} consecutive steps timed one by one on {threads} host thread(s), median "
+                      f"{med:.2f} s per step (steps: {', '.join(f'{t:.2f}' for t in ts)} s)"}
+
+
+def plain_leg(wl: str, n: int, steps: int, warmup: int, tile_arg: str):
+    """The fused time loop on the workload: wall-clock value and the stage
+    kernels' hipEvent timings."""
+    import torch
+    from pip_amd import autotune
+    from pip_amd.solver import PIPSolver
+    cfg, uh, um, ac = make_problem(wl, n)
+    if tile_arg == "auto":
+        tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
+    else:
+        tile = tuple(int(v) for v in tile_arg.split("x"))  # TXxTY[xREMAP[xTWO_PASS_MASK]]
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
+    s.enqueue(warmup)
+    s.sync()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    o.run(nsteps=steps)
-    dt = time.perf_counter() - t0
-    return {"value": cfg.ncells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C restatement of the reference Fortran), Orszag-Tang {n}x{n}, "
-                      f"{steps} steps, {dt:.1f} s on one core"}
+    ev = s.time_steps(steps)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = s.status()
+    s.close()
+    assert st == 0, f"stop_sim flags {st}"
+    ncell = cfg.ncells
+    nv = cfg.nvar_m + cfg.nvar_h
+    b_cell = cfg.bytes_per_cell_update()  # 5 nv 8 B (+ 16 B per coefficient field)
+    extra = (b_cell - 5 * nv * 8) // 2  # per stage
+    b1 = (2 * nv * 8 + extra) * ncell
+    b2 = (3 * nv * 8 + extra) * ncell
+    t1 = ev["stage1_ms"] / steps * 1e-3
+    t2 = ev["stage2_ms"] / steps * 1e-3
+    value = ncell * steps / wall
+    ach = (b1 + b2) / (t1 + t2) / 1e9
+    return cfg, tile, {
+        "value": value, "wall_s": wall, "ms_per_step": wall / steps * 1e3,
+        "ach_GBs": ach, "b1": b1, "b2": b2, "t1_ms": t1 * 1e3, "t2_ms": t2 * 1e3,
+        "b_cell": b_cell, "device_ms_per_step": ev["total_ms"] / steps,
+    }
+
+
+def measured_copy_gbs():
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    return 5 * 2 * 8 * (1 << 27) / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
 def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000):
@@ -136,7 +317,7 @@ def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000)
     ens = Ensemble(mem, tend=tend, max_steps=max_steps)
     (mode, tile), tune_ms = autotune.tune_ensemble(ens, f"ens:{n_members}:seed7:tend{tend}")
     ens.reset()
-    ens.run(graph=True)  # builds and caches the graph (persistent mode)
+    ens.run(graph=True)  # builds and caches the graph
     ens.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -236,16 +417,24 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kh",
+                    help="kh: Kelvin-Helmholtz with ionisation/recombination (headline); ot: Orszag-Tang")
+    ap.add_argument("--n", type=int, default=0, help="grid size n x n (default: the workload's)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=str, default="auto", help="TXxTY[xXCD_ORDER[xTWO_PASS_STAGE_MASK]], or auto (autotuner)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary, batched and adjoint legs")
     ap.add_argument("--pmc-summary", metavar="DIR", help="write the HBM-traffic profile summary from "
                     "rocprofv3 --pmc runs under DIR (fetch, write, copy_fetch, copy_write; the main "
                     "runs used --tile) and exit")
+    ap.add_argument("--stats-summary", nargs=3, metavar=("DIR", "OUT", "TITLE"),
+                    help="write a kernel-trace summary of the rocprofv3 --stats csv under DIR to OUT and exit")
+    ap.add_argument("--sq-summary", nargs=2, metavar=("DIR", "OUT"),
+                    help="write the SQ counter summary of the passes under DIR (sq1, sq2) to OUT and exit")
+    ap.add_argument("--stats-cmd", default="", help="command recorded in --stats-summary")
+    ap.add_argument("--stats-note", default="", help="note recorded in --stats-summary")
     ap.add_argument("--only", choices=["batched", "adjoint", "copy", "ensemble"],
                     help="run one extra leg alone (profiling); copy = 1 GiB device copy that "
                          "calibrates the HBM byte counters; ensemble = the batched leg's ensemble "
@@ -253,15 +442,22 @@ def main():
     ap.add_argument("--ens-mode", choices=["persistent", "lockstep"], default="lockstep")
     ap.add_argument("--ens-tile", type=int, choices=[64, 128, 256, 512], default=256)
     args = ap.parse_args()
+    n = args.n or WORKLOADS[args.workload][1]
+    if args.stats_summary:
+        d, out, title = args.stats_summary
+        write_stats_summary(d, args.stats_cmd, title, out, args.stats_note)
+        return
+    if args.sq_summary:
+        write_sq_summary(args.sq_summary[0], args.workload, n, args.tile, args.sq_summary[1])
+        return
     if args.pmc_summary:
-        pm = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
+        pm = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc {c} --output-format csv "
               "-d {d}/{sub} -o run -- python3 /root/repo/bench.py {a}")
-        tile = "32x7" if args.tile == "auto" else args.tile
-        main_args = f"--steps 20 --warmup 5 --no-cpu --no-extra --tile {tile}"
+        main_args = f"--workload {args.workload} --n {n} --steps 10 --warmup 3 --no-cpu --no-extra --tile {args.tile}"
         cmds = {sub: pm.format(c=c, d=args.pmc_summary, sub=sub, a=a) for sub, c, a in (
             ("fetch", "FETCH_SIZE", main_args), ("write", "WRITE_SIZE", main_args),
             ("copy_fetch", "FETCH_SIZE", "--only copy"), ("copy_write", "WRITE_SIZE", "--only copy"))}
-        write_pmc_summary(args.pmc_summary, cmds, tile)
+        write_pmc_summary(args.pmc_summary, cmds, args.workload, n, args.tile)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -269,17 +465,15 @@ def main():
         sys.exit(2)
 
     import torch  # device plumbing only: sync around the timed region
-    from pip_amd import autotune, problems
-    from pip_amd.solver import PIPSolver
 
     if args.only == "copy":
-        n = 1 << 27  # 2^27 doubles = 1 GiB per buffer
-        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        nn = 1 << 27  # 2^27 doubles = 1 GiB per buffer
+        src = torch.ones(nn, dtype=torch.float64, device="cuda:0")
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"copy": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n}}))
+        print(json.dumps({"copy": {"bytes_read_per_copy": 8 * nn, "bytes_written_per_copy": 8 * nn}}))
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble": ensemble_leg(args.ens_mode, args.ens_tile)}))
@@ -288,78 +482,53 @@ def main():
         print(json.dumps({args.only: batched_leg() if args.only == "batched" else adjoint_leg()}))
         return
 
-    cfg, uh, um, ac = problems.orszag_tang(args.n)
-    if args.tile == "auto":
-        tile, _ = autotune.tune_stage(lambda t: PIPSolver(cfg, uh, um, ac, tile=t), cfg)
-    else:
-        tile = tuple(int(v) for v in args.tile.split("x"))  # TXxTY[xREMAP[xTWO_PASS_MASK]]
-    s = PIPSolver(cfg, uh, um, ac, tile=tile)
-    s.enqueue(args.warmup)
-    s.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev = s.time_steps(args.steps)
-    s.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = s.status()
-    assert st == 0, f"stop_sim flags {st}"
-
-    # measured copy bandwidth on this GPU (1 GiB read + 1 GiB written)
-    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(5):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    copy_gbs = 5 * 2 * 8 * (1 << 27) / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del src, dst
-
-    ncell = cfg.ncells
-    value = ncell * args.steps / wall
-    nv = cfg.nvar_m + cfg.nvar_h
-    bytes_stage2 = 3 * nv * 8 * ncell
-    bytes_stage1 = 2 * nv * 8 * ncell
-    t2 = ev["stage2_ms"] / args.steps * 1e-3
-    t1 = ev["stage1_ms"] / args.steps * 1e-3
-    ach = bytes_stage2 / t2 / 1e9
-    traffic = traffic_from_profile(tile)
+    cfg, tile, r = plain_leg(args.workload, n, args.steps, args.warmup, args.tile)
+    copy_gbs = measured_copy_gbs()
+    traffic = traffic_from_profile(args.workload, n)
     out = {
         "metric": "FP64 two-fluid cell-updates/s",
-        "value": value,
+        "value": r["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (Orszag-Tang initial state, no checkpoint)",
-        "config": {"workload": "orszag_tang_2d_two_fluid", "nx": args.n, "ny": args.n,
-                   "nvar": nv, "xi_n": 0.9, "alpha_c": 1.0e3, "flag_divb": 1,
-                   "ssp_rk": 2, "cfl": 0.3, "tile": list(tile[:2]),
-                   "xcd_remap": int(tile[2]) if len(tile) > 2 else 0,
-                   "two_pass_stages": int(tile[3]) if len(tile) > 3 else 0},
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic[0], "traffic_tile": traffic[1],
-                     "kernel": ("k_stage2p" if len(tile) > 3 and (tile[3] >> 1) & 1 else "k_stage") + " (stage 2)",
-                     "algorithmic_bytes_per_launch": bytes_stage2,
-                     "avg_launch_ms": t2 * 1e3,
-                     "stage1_avg_launch_ms": t1 * 1e3,
-                     "stage1_achieved_GBs": bytes_stage1 / t1 / 1e9,
-                     "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": ach / copy_gbs},
-        "hbm_roofline_cell_updates_frac": value * (5 * nv * 8) / (HBM_PEAK_GBS * 1e9),
+        "data": f"synthetic ({WORKLOADS[args.workload][0]} initial state, seeded, no checkpoint)",
+        "config": workload_config(args.workload, n, cfg, tile),
+        "roofline": {"bound": "hbm", "achieved": r["ach_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["ach_GBs"] / HBM_PEAK_GBS,
+                     "traffic": traffic["mean"] if traffic else None,
+                     "kernel": "k_stage (both SSP-RK stages)",
+                     "algorithmic_bytes_per_launch": (r["b1"] + r["b2"]) / 2,
+                     "algorithmic_bytes_per_step": r["b1"] + r["b2"],
+                     "avg_launch_ms": (r["t1_ms"] + r["t2_ms"]) / 2,
+                     "stage1_ms": r["t1_ms"], "stage2_ms": r["t2_ms"],
+                     "traffic_stage1": traffic["stage1"] if traffic else None,
+                     "traffic_stage2": traffic["stage2"] if traffic else None,
+                     "traffic_tile": traffic["tile"] if traffic else None,
+                     "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": r["ach_GBs"] / copy_gbs},
+        "hbm_roofline_cell_updates_frac": r["value"] * r["b_cell"] / (HBM_PEAK_GBS * 1e9),
+        "bytes_per_cell_update": r["b_cell"],
+        "device_ms_per_step": r["device_ms_per_step"],
     }
     if not args.no_extra:
+        if args.workload == "kh":
+            c2, t2, r2 = plain_leg("ot", 1024, 500, 20, "auto")
+            out["secondary"] = {"config": workload_config("ot", 1024, c2, t2), "steps": 500,
+                                "value": r2["value"], "ms_per_step": r2["ms_per_step"],
+                                "roofline_achieved_GBs": r2["ach_GBs"],
+                                "roofline_frac": r2["ach_GBs"] / HBM_PEAK_GBS,
+                                "hbm_roofline_cell_updates_frac": r2["value"] * r2["b_cell"] / (HBM_PEAK_GBS * 1e9)}
         out["batched"] = batched_leg()
         out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.workload, n, args.cpu_steps, 1)
+        out["cpu_baseline_threads"] = cpu_baseline(args.workload, n, args.cpu_steps,
+                                                   min(16, os.cpu_count() or 1))
     print(json.dumps(out))
 
 

@@ -54,6 +54,7 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_void_p, P]
         L.orc_get_rates.argtypes = [ctypes.c_void_p, P, P, P, P]
         L.orc_set_rates.argtypes = [ctypes.c_void_p, P, P]
+        L.orc_set_arb_heat.argtypes = [ctypes.c_void_p, P]
         L.orc_get_scalars.argtypes = [ctypes.c_void_p, P]
         L.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_set_stage_save.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -72,6 +73,8 @@ def lib():
             getattr(L, name).restype = ctypes.c_double
         L.orc_pow.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_pow.restype = ctypes.c_double
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
         L.orc_minmod.argtypes = [ctypes.c_double] * 3
         L.orc_minmod.restype = ctypes.c_double
         L.orc_hllc_flux.argtypes = [P, P, ctypes.c_double, P]
@@ -148,6 +151,9 @@ class Oracle:
         lib().orc_set_rates(self.h, _p(np.ascontiguousarray(gm_rec, dtype=np.float64)),
                             _p(np.ascontiguousarray(gm_ion, dtype=np.float64)))
 
+    def set_arb_heat(self, arb_heat):
+        lib().orc_set_arb_heat(self.h, _p(np.ascontiguousarray(arb_heat, dtype=np.float64)))
+
     def fixed_rates(self, on=True):
         lib().orc_set_fixed_rates(self.h, int(on))
 
@@ -216,6 +222,12 @@ class Oracle:
         sm = np.zeros((c.nvar_m, c.jx, c.ix))
         lib().orc_source_pip(self.h, _p(sh), _p(sm))
         return sh, sm
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's elementwise loops (1 = serial); returns the
+    previous setting.  Results do not depend on it."""
+    return lib().orc_set_threads(int(n))
 
 
 def minmod(a, b, c) -> float:

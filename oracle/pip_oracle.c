@@ -12,9 +12,17 @@
  *     rewrites it (cq2pv_*, get_Pr_*).
  * Compile with -O2 -ffp-contract=off (no FMA contraction, like the reference
  * built for x86-64 without FMA).
+ *
+ * Elementwise loops (each iteration writes only its own cells) run on
+ * OpenMP threads when the grid has more than PAR_MIN cells; the min/max
+ * reductions of cfl stay serial, so every result is independent of the
+ * thread count.
  */
 #include "pip_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,6 +35,7 @@
 #define MU_N 1.0     /* PIP_rot.f90:37 */
 #define SGL(x) ((double)(float)(x))
 #define PI 3.1415926535897932 /* Parameters.f90:2 */
+#define PAR_MIN 16384         /* cells below which loops stay serial */
 
 struct orc_state {
     orc_cfg c;
@@ -177,6 +186,10 @@ void orc_set_rates(orc_state *s, const double *r, const double *i) {
     memcpy(s->gm_ion, i, sizeof(double) * s->N);
 }
 void orc_set_fixed_rates(orc_state *s, int on) { s->fixed_rates = on; }
+void orc_set_arb_heat(orc_state *s, const double *a) {
+    memcpy(s->arb_heat, a, sizeof(double) * s->N);
+    s->arb_set = 1;
+}
 void orc_get_scalars(const orc_state *s, double *o) {
     o[0] = s->dt; o[1] = s->time; o[2] = s->cmax; o[3] = (double)s->nt;
 }
@@ -187,6 +200,19 @@ void orc_set_scalars(orc_state *s, double dt, double cmax) {
 }
 
 void orc_set_stage_save(orc_state *s, double *buf) { s->stage_save = buf; }
+
+/* Threads of the elementwise loops (1: serial); returns the previous
+ * setting, or 1 when the library was built without OpenMP. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 /* -------------------------------------------------------------- limiter */
 /* minmod_func: Util_rot.f90:59-76 */
@@ -201,6 +227,7 @@ double orc_minmod(double U1, double U2, double U3) {
 static void u2v_mhd(const orc_state *s, const double *U, double *V) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double u1 = U[k], u2 = U[N + k], u3 = U[2 * N + k], u4 = U[3 * N + k];
         double u5 = U[4 * N + k], u6 = U[5 * N + k], u7 = U[6 * N + k], u8 = U[7 * N + k];
@@ -225,6 +252,7 @@ static void u2v_mhd(const orc_state *s, const double *U, double *V) {
 static void u2v_hd(const orc_state *s, const double *U, double *V) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double u1 = U[k], u2 = U[N + k], u3 = U[2 * N + k], u4 = U[3 * N + k];
         double u5 = U[4 * N + k];
@@ -261,6 +289,7 @@ static void v2u_1(const double *v1, double *u1, int flag0, int divb, double gm) 
 static void cq2pv_mhd(orc_state *s, double *U, double *prim) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double de = U[k];
         double vx = U[N + k] / de, vy = U[2 * N + k] / de, vz = U[3 * N + k] / de;
@@ -286,6 +315,7 @@ static void cq2pv_mhd(orc_state *s, double *U, double *prim) {
 static void cq2pv_hd(orc_state *s, double *U, double *prim) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double de = U[k];
         double vx = U[N + k] / de, vy = U[2 * N + k] / de, vz = U[3 * N + k] / de;
@@ -308,6 +338,7 @@ static void cq2pv_hd(orc_state *s, double *U, double *prim) {
 static void get_pr_hd(orc_state *s, double *U, double *pr_out, double *te_out) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double u1 = U[k], u2 = U[N + k], u3 = U[2 * N + k], u4 = U[3 * N + k];
         double pr = (c->gm - 1.0) * (U[4 * N + k] - (0.5 * ((u2 * u2 + u3 * u3) + u4 * u4)) / u1);
@@ -322,6 +353,7 @@ static void get_pr_hd(orc_state *s, double *U, double *pr_out, double *te_out) {
 static void get_pr_mhd(orc_state *s, double *U, double *pr_out, double *te_out) {
     const orc_cfg *c = &s->c;
     int N = s->N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double u1 = U[k], u2 = U[N + k], u3 = U[2 * N + k], u4 = U[3 * N + k];
         double u6 = U[5 * N + k], u7 = U[6 * N + k], u8 = U[7 * N + k];
@@ -345,6 +377,7 @@ static void get_side(const orc_state *s, const double *V, double *VL, double *VR
         const double *U = V + (size_t)n * N;
         double *L = VL + (size_t)n * N, *R = VR + (size_t)n * N;
         if (dir == 1) {
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
             for (int j = 0; j < jx; j++)
                 for (int i = 1; i <= ix - 2; i++) {
                     int k = j * ix + i;
@@ -353,6 +386,7 @@ static void get_side(const orc_state *s, const double *V, double *VL, double *VR
                     R[k - 1] = U[k] - 0.5 * du;
                 }
         } else {
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
             for (int j = 1; j <= jx - 2; j++)
                 for (int i = 0; i < ix; i++) {
                     int k = j * ix + i;
@@ -362,6 +396,7 @@ static void get_side(const orc_state *s, const double *V, double *VL, double *VR
                 }
         }
     }
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         VL[k] = dmax(VL[k], c->ro_lim);
         VR[k] = dmax(VR[k], c->ro_lim);
@@ -586,21 +621,24 @@ static void hlld_fluxes_ideal(orc_state *s, double *F) {
     for (int dir = 1; dir <= c->ndim; dir++) {
         u2v_mhd(s, s->Um, s->V);
         get_side(s, s->V, s->VL, s->VR, dir, nv);
-        double vl[9], vr[9], fl[9];
         double *Fd = F + (size_t)(dir - 1) * nv * N;
         if (dir == 1) {
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
             for (int j = 0; j < jx; j++)
                 for (int i = 0; i <= ix - 2; i++) {
                     int k = j * ix + i;
+                    double vl[9], vr[9], fl[9];
                     for (int n = 0; n < nv; n++) { vl[n] = s->VL[n * N + k]; vr[n] = s->VR[n * N + k]; }
                     orc_hlld_flux(vl, vr, nv, c->flag_divb, s->cmax, c->gm, fl);
                     for (int n = 0; n < nv; n++) Fd[n * N + k] = fl[n];
                 }
         } else {
             static const int perm[9] = {0, 2, 1, 3, 4, 6, 5, 7, 8};
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
             for (int j = 0; j <= jx - 2; j++)
                 for (int i = 0; i < ix; i++) {
                     int k = j * ix + i;
+                    double vl[9], vr[9], fl[9];
                     for (int n = 0; n < nv; n++) {
                         vl[n] = s->VL[perm[n] * N + k];
                         vr[n] = s->VR[perm[n] * N + k];
@@ -620,14 +658,15 @@ static void hllc_fluxes_ideal(orc_state *s, double *F) {
     for (int dir = 1; dir <= c->ndim; dir++) {
         u2v_hd(s, s->Uh, s->V);
         get_side(s, s->V, s->VL, s->VR, dir, 5);
-        double vl[5], vr[5], fl[5];
         double *Fd = F + (size_t)(dir - 1) * 5 * N;
         static const int perm[5] = {0, 2, 1, 3, 4};
         int imax = (dir == 1) ? ix - 2 : ix - 1;
         int jmax = (dir == 1) ? jx - 1 : jx - 2;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
         for (int j = 0; j <= jmax; j++)
             for (int i = 0; i <= imax; i++) {
                 int k = j * ix + i;
+                double vl[5], vr[5], fl[5];
                 for (int n = 0; n < 5; n++) {
                     int p = (dir == 1) ? n : perm[n];
                     vl[n] = s->VL[p * N + k];
@@ -724,6 +763,7 @@ void orc_set_ir(orc_state *s) {
     double tfac;
     if (c->flag_col % 2 == 1) tfac = 0.5 * f_p_p / f_p;
     else tfac = c->beta / 2.0 * f_p_p * 5.0 / 6.0 / f_p;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double rho = s->Um[k];
         s->gm_rec[k] = rho / sqrt(Te_p[k]) * c->t_ir / f_p * sqrt(tfac);
@@ -757,6 +797,7 @@ static void set_collisional(orc_state *s) {
     double *Te_m = s->tmp, *Te_h = s->tmp + N;
     get_pr_mhd(s, s->Um, NULL, Te_m); /* get_Te_MHD :57 */
     get_pr_hd(s, s->Uh, NULL, Te_h);  /* get_Te_HD  :58 */
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double tt = Te_h[k] + Te_m[k];
         double a = c->col * sqrt(tt / 2.0);
@@ -869,6 +910,7 @@ void orc_source_pip(orc_state *s, double *S_h, double *S_m) {
     memset(S_m, 0, sizeof(double) * N * nv);
     cq2pv_hd(s, s->Uh, ph);
     cq2pv_mhd(s, s->Um, pmx);
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
     for (int k = 0; k < N; k++) {
         double nde = ph[k], nvx = ph[N + k], nvy = ph[2 * N + k], nvz = ph[3 * N + k], npr = ph[4 * N + k];
         double de = pmx[k], vx = pmx[N + k], vy = pmx[2 * N + k], vz = pmx[3 * N + k], pr = pmx[4 * N + k];
@@ -954,6 +996,7 @@ static void add_flux_ssprk(orc_state *s, const double *F, const double *U0, doub
     for (int n = 0; n < nvar; n++) {
         const double *f = Fx + (size_t)n * N, *u0 = U0 + (size_t)n * N;
         double *u = U + (size_t)n * N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
         for (int j = 0; j < jx; j++)
             for (int i = 1; i <= ix - 1; i++) {
                 int k = j * ix + i;
@@ -966,6 +1009,7 @@ static void add_flux_ssprk(orc_state *s, const double *F, const double *U0, doub
         for (int n = 0; n < nvar; n++) {
             const double *f = Fy + (size_t)n * N;
             double *u = U + (size_t)n * N;
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
             for (int j = 1; j <= jx - 1; j++)
                 for (int i = 0; i < ix; i++) {
                     int k = j * ix + i;
@@ -995,8 +1039,10 @@ void orc_run_solver(orc_state *s) {
             dt_sub = c1 * s->dt;
         }
         add_flux_ssprk(s, s->Fh, s->Uh0, s->Uh, 5, dt_sub, c0, c1, istep);
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
         for (int k = 0; k < N * 5; k++) s->Uh[k] = s->Uh[k] + s->Sh[k] * dt_sub; /* add_source :498 */
         add_flux_ssprk(s, s->Fm, s->Um0, s->Um, nv, dt_sub, c0, c1, istep);
+#pragma omp parallel for schedule(static) if (N > PAR_MIN)
         for (int k = 0; k < N * nv; k++) s->Um[k] = s->Um[k] + s->Sm[k] * dt_sub;
         if (c->flag_divb == 1) { /* divb_cleaning_Dedner_source Scheme_rot.f90:546-554 */
             double f = orc_exp(-dt_sub / s->dt * c->safety * c->db_clean);

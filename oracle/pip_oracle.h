@@ -66,6 +66,12 @@ int orc_step(orc_state *s);                    /* main.f90:31-68, one pass */
 int orc_stop_sim(const orc_state *s);          /* IO_rot.f90:625-674       */
 /* fixed rates mode for source tests (no set_IR): IR sources use given rates */
 void orc_set_fixed_rates(orc_state *s, int on);
+/* test hook: arb_heat field (as if set_IR had fixed it) */
+void orc_set_arb_heat(orc_state *s, const double *arb_heat);
+
+/* threads of the elementwise loops (OpenMP; 1 = serial); returns the
+ * previous setting */
+int orc_set_threads(int n);
 
 /* point functions (pure) */
 double orc_exp(double x);

@@ -121,7 +121,9 @@ class TwoFluidRun:
 
     # ------------------------------------------------------------ passes
     def forward_record(self, U0: torch.Tensor, ac: torch.Tensor):
-        """Forward run recording dt/cmax; returns (U_T, checkpoints)."""
+        """Forward run recording dt/cmax; returns (U_T, checkpoints, history),
+        history being this run's own copy of the recorded (dt, cmax) pairs
+        (the runner's buffer is overwritten by the next forward)."""
         self._start(U0, ac)
         L = lib()
         ckpts = []
@@ -133,22 +135,28 @@ class TwoFluidRun:
         out = self._empty_state()
         check(L.pip_get_state_device(self.h, _ptr(out)))
         self._lib_done()
-        return out, ckpts
+        return out, ckpts, self.hist.clone()
 
-    def replay(self, U0: torch.Tensor, ac: torch.Tensor) -> torch.Tensor:
-        """The same run with the recorded dt/cmax sequence held fixed (the map
-        whose derivative backward() returns)."""
+    def replay(self, U0: torch.Tensor, ac: torch.Tensor, hist: torch.Tensor | None = None) -> torch.Tensor:
+        """The same run with a recorded dt/cmax sequence held fixed (the map
+        whose derivative backward() returns); hist defaults to the last
+        recorded run's."""
+        hist = self.hist if hist is None else hist
         self._start(U0, ac)
+        self._torch_done()
         self._use_tile(self.replay_tile)
-        check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(self.hist), None, None, None, None))
+        check(lib().pip_replay_steps(self.h, self.nsteps, _ptr(hist), None, None, None, None))
         self._use_tile(self.tile)
         out = self._empty_state()
         check(lib().pip_get_state_device(self.h, _ptr(out)))
         self._lib_done()
         return out
 
-    def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor):
-        """dJ/dU0, dJ/dac from dJ/dU_T by checkpointed reverse sweeps."""
+    def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor, hist: torch.Tensor):
+        """dJ/dU0, dJ/dac from dJ/dU_T by checkpointed reverse sweeps over the
+        forward run that produced ckpts and hist."""
+        if hist.numel() < 2 * self.nsteps or len(ckpts) != len(self._segments()):
+            raise ValueError("checkpoints and history do not belong to a run of this runner")
         L = lib()
         lam = gU.detach().to(torch.float64).contiguous().clone()
         lam_ac = torch.zeros_like(ac)
@@ -169,14 +177,14 @@ class TwoFluidRun:
         self._use_tile(self.replay_tile)
         for (k0, m), ck in zip(reversed(segs), reversed(ckpts)):
             check(L.pip_set_state_device(self.h, _ptr(ck)))
-            check(L.pip_replay_steps(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
+            check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(self._seg_Un),
                                      _ptr(self._seg_U1), _ptr(self._seg_Vn) if sv else None,
                                      _ptr(self._seg_V1) if sv else None))
             for k in reversed(range(m)):
                 check(L.pip_adj_step(self.h, _ptr(self._seg_Un) + sb * k, _ptr(self._seg_U1) + sb * k,
                                      _ptr(self._seg_Vn) + sb * k if sv else None,
                                      _ptr(self._seg_V1) + sb * k if sv else None,
-                                     _ptr(self.hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
+                                     _ptr(hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
         self._use_tile(self.tile)
         self._lib_done()
         return lam, lam_ac
@@ -198,16 +206,18 @@ class _Evolve(torch.autograd.Function):
     def forward(ctx, U0, ac, runner: TwoFluidRun):
         U0c = U0.detach().contiguous()
         acc = ac.detach().contiguous()
-        out, ckpts = runner.forward_record(U0c, acc)
+        out, ckpts, hist = runner.forward_record(U0c, acc)
         ctx.runner = runner
         ctx.ckpts = ckpts
-        ctx.save_for_backward(acc)
+        # this call's own dt/cmax sequence: a later forward on the same
+        # runner overwrites runner.hist, not this copy
+        ctx.save_for_backward(acc, hist)
         return out
 
     @staticmethod
     def backward(ctx, gU):
-        (acc,) = ctx.saved_tensors
-        lam, lam_ac = ctx.runner.backward(gU, ctx.ckpts, acc)
+        acc, hist = ctx.saved_tensors
+        lam, lam_ac = ctx.runner.backward(gU, ctx.ckpts, acc, hist)
         return lam, lam_ac, None
 
 

@@ -1,7 +1,8 @@
 // hostcheck.cpp -- host build of the per-face/per-cell device functions
 // (pip_phys.h, pip_adj_gen.h, pip_adj_point.h) for CPU tests: forward
 // fluxes, reconstruction and conversions, and their vector-Jacobian
-// products.  Test infrastructure; not used by libpipgpu.
+// products.  Test infrastructure (built by tests/test_adjoint_cpu.py with
+// -I pip_amd/csrc); not used by libpipgpu.
 #include <vector>
 
 #include "pip_adj_gen.h"

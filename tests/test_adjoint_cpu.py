@@ -20,14 +20,13 @@ P = ctypes.POINTER(ctypes.c_double)
 
 
 def _build():
-    src = os.path.join(build.CSRC, "hostcheck.cpp")
-    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
-            os.path.getmtime(os.path.join(build.CSRC, f))
-            for f in ("hostcheck.cpp", "pip_phys.h", "pip_adj_gen.h", "pip_adj_point.h", "pip_bnd.h",
-                      "pip_adj_items.h")):
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostcheck.cpp")
+    deps = [src] + [os.path.join(build.CSRC, f) for f in ("pip_phys.h", "pip_adj_gen.h", "pip_adj_point.h",
+                                                          "pip_bnd.h", "pip_adj_items.h")]
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in deps):
         os.makedirs(build.LIBDIR, exist_ok=True)
         subprocess.check_call(["g++", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-std=c++17",
-                               "-o", LIB, src])
+                               "-I" + build.CSRC, "-o", LIB, src])
     L = ctypes.CDLL(LIB)
     D = ctypes.c_double
     L.hc_hllc.argtypes = [P, P, D, P]

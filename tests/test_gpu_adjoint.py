@@ -86,7 +86,7 @@ def test_replay_is_bitwise_the_recorded_run():
     print("case replay vs record n=64 steps=45", flush=True)
     cfg, run, base, v0, ac, _ = _setup(64, 45, 7)
     U0 = build_U0(cfg, base, v0)
-    out, ckpts = run.forward_record(U0, ac)
+    out, ckpts, hist = run.forward_record(U0, ac)
     assert len(ckpts) == 7
     rep = run.replay(U0, ac)
     assert torch.equal(out, rep)
@@ -100,7 +100,7 @@ def test_record_matches_fused_loop():
     dt sequence); only the final rewrite round differs (ulp level)."""
     print("case record vs pip_advance n=64 steps=30", flush=True)
     cfg, run, base, v0, ac, (uh, um, acn) = _setup(64, 30, 10)
-    out, _ = run.forward_record(build_U0(cfg, base, v0), ac)
+    out, _, _ = run.forward_record(build_U0(cfg, base, v0), ac)
     s = PIPSolver(cfg, uh, um, acn)
     s.advance(30)
     uh_T, um_T = s.state()
@@ -124,7 +124,7 @@ def test_saved_primitives_match_recomputed():
         cfg, run, base, v0, ac, _ = _setup(48, 12, 5)
         run.save_primitives = save
         U0 = build_U0(cfg, base, v0)
-        out, ckpts = run.forward_record(U0, ac)
+        out, ckpts, hist = run.forward_record(U0, ac)
         if save:
             bufs = [run._empty_state(12) for _ in range(4)]
             run._start(U0, ac)
@@ -140,7 +140,7 @@ def test_saved_primitives_match_recomputed():
                 t = t + float(dt)
             assert s1["time"] == t and s1["nt"] == s0["nt"] + 12
         g = torch.sin(out * 3.0)
-        grads.append(run.backward(g, ckpts, ac))
+        grads.append(run.backward(g, ckpts, ac, hist))
         run.close()
     assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
 
@@ -156,8 +156,8 @@ def test_two_pass_stage_kernels_give_the_same_gradient():
         from pip_amd.adjoint import TwoFluidRun
         run = TwoFluidRun(cfg, 12, ckpt_every=5, tile=tile, replay_tile=tile)
         U0 = build_U0(cfg, base, v0)
-        out, ckpts = run.forward_record(U0, ac)
-        lam, lam_ac = run.backward(torch.cos(out), ckpts, ac)
+        out, ckpts, hist = run.forward_record(U0, ac)
+        lam, lam_ac = run.backward(torch.cos(out), ckpts, ac, hist)
         outs.append((out, lam, lam_ac))
         run.close()
     for a, b in zip(*outs):
@@ -175,10 +175,43 @@ def test_replay_tile_differs_from_the_recording_tile():
         run0.close()
         run = TwoFluidRun(cfg, 12, ckpt_every=5, tile=tile, replay_tile=rt)
         assert run.replay_tile == (16, 15, 1, 0)
-        out, ckpts = run.forward_record(build_U0(cfg, base, v0), ac)
-        outs.append((out,) + run.backward(torch.cos(out), ckpts, ac))
+        out, ckpts, hist = run.forward_record(build_U0(cfg, base, v0), ac)
+        outs.append((out,) + run.backward(torch.cos(out), ckpts, ac, hist))
         run.close()
     for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+def test_two_forwards_then_one_backward():
+    """Each differentiable run keeps its own recorded dt sequence: two
+    forwards on one runner (different alpha fields, hence different dt
+    sequences), then the first one's backward, give the gradient of the
+    first run done alone."""
+    print("case adjoint two forwards, one backward n=48 steps=12", flush=True)
+    from pip_amd.adjoint import TwoFluidRun
+    cfg, run, base, v0, ac, _ = _setup(48, 12, 5)
+    U0 = build_U0(cfg, base, v0)
+
+    def grads(runner, a, others=()):
+        U = U0.clone().requires_grad_(True)
+        av = a.clone().requires_grad_(True)
+        UT = runner.run(U, av)
+        for o in others:  # later forwards on the same runner
+            runner.run(U0.clone().requires_grad_(True), o.clone().requires_grad_(True))
+        (UT * torch.cos(UT)).sum().backward()
+        return UT.detach(), U.grad, av.grad
+
+    ref = grads(run, ac)
+    run.close()
+    run = TwoFluidRun(cfg, 12, ckpt_every=5)
+    got = grads(run, ac, others=(2.0 * ac,))
+    h_other = run.dt_history()
+    run.close()
+    run = TwoFluidRun(cfg, 12, ckpt_every=5)
+    run.run(U0.clone(), ac.clone())
+    assert not np.array_equal(run.dt_history(), h_other)  # the second run's dt sequence differs
+    run.close()
+    for a, b in zip(ref, got):
         assert torch.equal(a, b)
 
 

@@ -5,6 +5,13 @@ These are written from the published methods, not from the reference code:
   - HLLC in Toro's star-state form (Toro, Riemann Solvers and Numerical
     Methods for Fluid Dynamics, 3rd ed., eqs. 10.33-10.39);
   - HLLD of Miyoshi & Kusano (2005, J. Comput. Phys. 208, 315), eqs. 38-63;
+  - HLLC for ideal MHD of Li (2005, J. Comput. Phys. 203, 344), with the
+    HLL-averaged transverse velocity and magnetic field in both intermediate
+    states (the "HLLC-G" form of Gurski 2004, SIAM J. Sci. Comput. 25, 2165);
+  - the momentum-transfer rate of hard-sphere collisions between two
+    drifting Maxwellian species and their mean relative speed (kinetic
+    theory; e.g. Draine 1986, MNRAS 220, 133, appendix), in closed form and
+    by quadrature;
   - the exact Riemann solver for the Euler equations (Toro ch. 4);
   - Voronov (1997) hydrogen ionisation fit and the radiative recombination
     rate 2.6e-19 T_eV^-1/2 m^3/s.
@@ -156,6 +163,120 @@ def hlld_fallback_triggered(vl, vr, gm, factor=1.001):
         return True
     sal, sar = abs(bx) / math.sqrt(rsl), abs(bx) / math.sqrt(rsr)
     return (sl >= sm - factor * sal) or (sm + factor * sar >= sr)
+
+
+def li_hllc_mhd(vl, vr, gm):
+    """HLLC for ideal MHD (Li 2005) with the Miyoshi-Kusano wave-speed
+    estimates: one contact wave S_M = (rho u)_hll / rho_hll; in the
+    intermediate state K* (K = L, R):
+        rho* = rho_K (S_K - u_K) / (S_K - S_M),  (rho u)* = rho* S_M,
+        (rho v_t)* = rho* v_t,hll,  B* = B_hll,
+        p_T* = p_T,K + rho_K (S_K - u_K)(S_M - u_K),
+        E* = [(S_K - u_K) E_K - p_T,K u_K + p_T* S_M + B_x (v_K.B_K - v*.B*)] / (S_K - S_M),
+    and F = F_K + S_K (U*_K - U_K) on the side of the contact that holds
+    x/t = 0.  Constant normal field (B_x,L = B_x,R) assumed."""
+    assert vl[5] == vr[5]
+    cfl, cfr = fast_speed(vl, gm), fast_speed(vr, gm)
+    sl = min(vl[1], vr[1]) - max(cfl, cfr)
+    sr = max(vl[1], vr[1]) + max(cfl, cfr)
+    Ul, Ur = mhd_cons(vl, gm), mhd_cons(vr, gm)
+    Fl, Fr = mhd_flux(vl, gm), mhd_flux(vr, gm)
+    if sl >= 0:
+        return Fl
+    if sr <= 0:
+        return Fr
+    Uh = (sr * Ur - sl * Ul - Fr + Fl) / (sr - sl)
+    sm = Uh[1] / Uh[0]
+    vt = Uh[2:4] / Uh[0]
+    Bh = Uh[5:8]
+
+    def star(v, U, s):
+        r, u = v[0], v[1]
+        pt = v[4] + 0.5 * (v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
+        pts = pt + r * (s - u) * (sm - u)
+        rs = r * (s - u) / (s - sm)
+        vb = u * v[5] + v[2] * v[6] + v[3] * v[7]
+        vbs = sm * Bh[0] + vt[0] * Bh[1] + vt[1] * Bh[2]
+        es = ((s - u) * U[4] - pt * u + pts * sm + Bh[0] * (vb - vbs)) / (s - sm)
+        return np.array([rs, rs * sm, rs * vt[0], rs * vt[1], es, Bh[0], Bh[1], Bh[2]])
+
+    if sm >= 0:
+        return Fl + sl * (star(vl, Ul, sl) - Ul)
+    return Fr + sr * (star(vr, Ur, sr) - Ur)
+
+
+def smooth_switch(s, eps):
+    """The reference HLLC's upwind switches written piecewise: 0 for s <= 0,
+    the ramp 2 s / eps on 0 < s <= eps/2, 1 above."""
+    if s <= 0:
+        return 0.0
+    if s <= 0.5 * eps:
+        return 2.0 * s / eps
+    return 1.0
+
+
+def hllc_blend(vl, vr, gm, eps=1.0e-10, acc=1.03):
+    """HLLC with the smoothed upwind switches: with th_L = sw(S_L),
+    th_M = sw(S_M), th_R = 1 - sw(S_R) (sw = smooth_switch),
+        F = th_R F_R + th_L F_L + (1 - th_L - th_R)(th_M F*_L + (1 - th_M) F*_R),
+    F*_K = F_K + S_K (U*_K - U_K) with Toro's star states (eq. 10.39).
+    Returns (F, th_L, th_M, th_R)."""
+    rl, ul, _, _, pl = vl
+    rr, ur, _, _, pr = vr
+    cl, cr = math.sqrt(gm * pl / rl), math.sqrt(gm * pr / rr)
+    sl = acc * min(ul - cl, ur - cr)
+    sr = acc * max(ul + cl, ur + cr)
+    ss = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur))
+    Ul, Ur = euler_cons(vl, gm), euler_cons(vr, gm)
+    Fl, Fr = euler_flux(vl, gm), euler_flux(vr, gm)
+
+    def star(v, U, s):
+        r, u, vv, w, p = v
+        f = r * (s - u) / (s - ss)
+        return f * np.array([1.0, ss, vv, w, U[4] / r + (ss - u) * (ss + p / (r * (s - u)))])
+    Fsl = Fl + sl * (star(vl, Ul, sl) - Ul)
+    Fsr = Fr + sr * (star(vr, Ur, sr) - Ur)
+    thl, thm, thr = smooth_switch(sl, eps), smooth_switch(ss, eps), 1.0 - smooth_switch(sr, eps)
+    F = thr * Fr + thl * Fl + (1.0 - thl - thr) * (thm * Fsl + (1.0 - thm) * Fsr)
+    return F, thl, thm, thr
+
+
+# --------------------------------------------------------------------------
+def drift_drag_rate(x):
+    """Momentum-transfer rate of hard-sphere collisions per unit relative
+    drift, for a relative velocity distributed as exp(-|w - u|^2 / a^2)
+    (two Maxwellian species, a^2 = 2 k (T_1/m_1 + T_2/m_2)), in units of
+    sigma a, at x = |u| / a:  <|w| w.u_hat> / (a x) with
+        <|w| w.u_hat> = a^2 [ (x + 1/(2x)) e^{-x^2} / sqrt(pi) + (x^2 + 1 - 1/(4x^2)) erf(x) ].
+    Limits: 8 / (3 sqrt(pi)) as x -> 0, x as x -> infinity."""
+    if x < 1e-3:  # series: the closed form cancels 1/x terms here
+        return 8.0 / (3.0 * math.sqrt(math.pi)) * (1.0 + 0.2 * x * x)
+    R = (x + 0.5 / x) * math.exp(-x * x) / math.sqrt(math.pi) + (x * x + 1.0 - 0.25 / (x * x)) * math.erf(x)
+    return R / x
+
+
+def drift_drag_rate_quad(x):
+    """drift_drag_rate by direct quadrature over the shifted Maxwellian
+    (w in [0, x + 12], cos(theta) in [-1, 1]), a = 1."""
+    from scipy import integrate
+    f = lambda mu, w: w ** 4 * mu * math.exp(-(w * w - 2 * w * x * mu + x * x))
+    val, _ = integrate.dblquad(f, 0.0, x + 12.0, -1.0, 1.0, epsabs=1e-13, epsrel=1e-11)
+    return 2.0 / math.sqrt(math.pi) * val / x
+
+
+def mean_relative_speed(x):
+    """<|w|> / a for the same distribution: (x + 1/(2x)) erf(x) + e^{-x^2}/sqrt(pi);
+    2/sqrt(pi) as x -> 0, x as x -> infinity."""
+    if x < 1e-4:
+        return 2.0 / math.sqrt(math.pi) * (1.0 + x * x / 3.0)
+    return (x + 0.5 / x) * math.erf(x) + math.exp(-x * x) / math.sqrt(math.pi)
+
+
+def mean_relative_speed_quad(x):
+    from scipy import integrate
+    f = lambda mu, w: w ** 3 * math.exp(-(w * w - 2 * w * x * mu + x * x))
+    val, _ = integrate.dblquad(f, 0.0, x + 12.0, -1.0, 1.0, epsabs=1e-13, epsrel=1e-11)
+    return 2.0 / math.sqrt(math.pi) * val
 
 
 # --------------------------------------------------------------------------
