@@ -143,6 +143,24 @@ def mk_hlld(vl, vr, gm):
     return Fr + ssr * U2 - (ssr - sr) * Usr - sr * Ur, "R**"
 
 
+def hlld_fallback_margin(vl, vr, gm):
+    """The factor f at which the fallback condition S_L >= S_M - f S_A,L or
+    S_M + f S_A,R >= S_R starts to hold: min((S_M - S_L)/S_A,L,
+    (S_R - S_M)/S_A,R) (subsonic pairs with positive star densities)."""
+    bx = vl[5]
+    cfl, cfr = fast_speed(vl, gm), fast_speed(vr, gm)
+    sl = min(vl[1], vr[1]) - max(cfl, cfr)
+    sr = max(vl[1], vr[1]) + max(cfl, cfr)
+
+    def ptot(v):
+        return v[4] + 0.5 * (v[5] ** 2 + v[6] ** 2 + v[7] ** 2)
+    rl, ul, rr, ur = vl[0], vl[1], vr[0], vr[1]
+    sm = ((sr - ur) * rr * ur - (sl - ul) * rl * ul - ptot(vr) + ptot(vl)) / ((sr - ur) * rr - (sl - ul) * rl)
+    rsl = rl * (sl - ul) / (sl - sm)
+    rsr = rr * (sr - ur) / (sr - sm)
+    return min((sm - sl) * math.sqrt(rsl) / abs(bx), (sr - sm) * math.sqrt(rsr) / abs(bx))
+
+
 def hlld_fallback_triggered(vl, vr, gm, factor=1.001):
     """True when the HLLC-G fallback condition of the OpenMHD formulation
     (S_L >= S_M - factor*S_A,L or S_M + factor*S_A,R >= S_R) holds."""
