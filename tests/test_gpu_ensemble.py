@@ -1,8 +1,10 @@
 """Batched ensemble (one persistent launch, optional hipGraph replay, or
 lockstep launches with every member spread over workgroups) equals the plain
-per-member runs bit for bit."""
+per-member runs and the oracle bit for bit."""
 import numpy as np
 import pytest
+
+from oracle.oracle import Oracle
 
 from pip_amd import problems
 from pip_amd.ensemble import Ensemble
@@ -106,3 +108,39 @@ def test_lockstep_and_persistent_agree_on_48_members():
         assert ens.result(k) == r, k
         eh, em = ens.state(k)
         assert np.array_equal(eh, uh) and np.array_equal(em, um), k
+
+
+@pytest.mark.timeout(600)
+def test_ensemble_members_match_the_oracle():
+    """Members of the 256-member ensemble (seed 7) run to t = 0.1 inside a
+    24-member ensemble, both launch modes, against the oracle's runs of the
+    same members: the stiffest (alpha_c = 9.8e3), the largest (nx = 4089),
+    the smallest (nx = 260) and two others -- every interior cell, the step
+    count and the end time bitwise."""
+    print("case ensemble vs oracle: members 175, 98, 212, 3, 100 in a 24-member ensemble", flush=True)
+    params = problems.ensemble_members(256, seed=7)
+    check = [175, 98, 212, 3, 100]
+    assert params[175]["col"] > 9.7e3 and params[98]["nx"] == max(p["nx"] for p in params)
+    pick = check + [k for k in range(19)]
+    mem = [problems.ensemble_member(params[k])[:3] for k in pick]
+    ref = []
+    for k in range(len(check)):
+        cfg, uh, um = mem[k]
+        o = Oracle(cfg, uh, um)
+        assert o.run(nsteps=MAXS, tend=0.1) == 0
+        ref.append((o.scalars(), o.state()))
+        print(f"  oracle member {check[k]}: nx {cfg.nx}, {ref[-1][0]['nt']} steps", flush=True)
+    ens = Ensemble(mem, tend=0.1, max_steps=MAXS, tile=256)
+    for mode in ("persistent", "lockstep"):
+        ens.set_mode(mode)
+        ens.reset()
+        ens.run(graph=True)
+        for k in range(len(check)):
+            sc, (oh, om) = ref[k]
+            r = ens.result(k)
+            assert r["status"] == 0 and r["nt"] == sc["nt"] and r["time"] == sc["time"], (mode, check[k], r, sc)
+            eh, em = ens.state(k)
+            assert np.array_equal(em[:, :, 2:-2], om[:, :, 2:-2]), (mode, check[k])
+            assert np.array_equal(eh[:, :, 2:-2], oh[:, :, 2:-2]), (mode, check[k])
+        print(f"  {mode}: bitwise equal", flush=True)
+    ens.close()
