@@ -159,6 +159,21 @@ int pip_adj_seed(pip_ctx *ctx, double *lam);
 int pip_adj_step(pip_ctx *ctx, const double *Un, const double *U1, const double *Vn, const double *V1,
                  const double *dtc, double *lam, double *lam_ac);
 
+/* pip_adj_step without allocation.  No counterpart in the reference (PIP
+ * has no derivatives); for callers that cannot allocate device memory --
+ * code under HIP graph capture, tight loops of an inverse solver, harnesses
+ * that forbid hipMalloc.  pip_adj_step allocates its work buffers on its
+ * first call; this variant keeps them (the face-state adjoints and branch
+ * codes, and the adjoint of the stage-1 result: (5 (nvar_m + 5) + 4) * 8
+ * bytes per padded cell, 592 B with psi) in one fixed 64 MiB device array
+ * per GPU, so grids are limited to ix * jx <= 113359 (up to 332^2 interior
+ * cells); larger grids return PIP_ERR_ARG.  Vn and V1 (the replay's saved
+ * primitives) are required.  Same kernels and results as pip_adj_step.
+ * Every call on one GPU uses that GPU's array: calls that overlap on one
+ * GPU (other streams, other contexts) race. */
+int pip_adj_step_noalloc(pip_ctx *ctx, const double *Un, const double *U1, const double *Vn,
+                         const double *V1, const double *dtc, double *lam, double *lam_ac);
+
 /* ---- batched ensemble (no counterpart in the reference: PIP runs one
  * domain per process).  Independent 1-D two-fluid runs of different sizes,
  * each with its own dt/time/step count, integrated in ONE persistent launch

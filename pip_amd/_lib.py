@@ -98,6 +98,7 @@ _SIGS = {
     "pip_replay_steps": [_V, _I, _V, _V, _V, _V, _V],
     "pip_adj_seed": [_V, _V],
     "pip_adj_step": [_V, _V, _V, _V, _V, _V, _V, _V],
+    "pip_adj_step_noalloc": [_V, _V, _V, _V, _V, _V, _V, _V],
     "pip_ens_create": [ctypes.POINTER(PipCfg), ctypes.POINTER(PipEnsMember), _I, _PP],
     "pip_ens_upload": [_V, _I, _P, _P],
     "pip_ens_download": [_V, _I, _P, _P],

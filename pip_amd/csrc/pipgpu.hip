@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,34 @@ __global__ void k_check_arith(const double* n, const double* d, double* q, doubl
   if (k >= count) return;
   q[k] = qdiv(n[k], d[k]);
   r[k] = qsqrt(n[k]);
+}
+
+// Work memory of pip_adj_step_noalloc: the face-state adjoint pairs and
+// branch codes of both directions and the adjoint of the stage-1 result,
+// (5 nvt + 4) doubles per padded cell, in one fixed device array per GPU
+// (held until the process ends).  Host code never passes the array's name
+// to a kernel: it is a host-side stand-in for the symbol.  The device
+// address comes from hipGetSymbolAddress, taken once per device when the
+// first context is created on it (pip_create).
+constexpr size_t kAdjScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kAdjScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// Once per device, with that device current: the device address of scratch.
+static void init_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[dev]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_ptr[dev] = (e == hipSuccess) ? static_cast<double*>(ptr) : nullptr;
+  g_scratch_err[dev] = e;
+  g_scratch_done[dev] = true;
 }
 
 struct pip_ctx {
@@ -380,6 +409,7 @@ int pip_create(const pip_cfg* cfg, pip_ctx** out) {
     return fail(PIP_ERR_HIP, m);
   };
   if ((e = hipGetDevice(&c->device)) != hipSuccess) return cleanup("hipGetDevice");
+  init_scratch(c->device);
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return cleanup("hipStreamCreate");
   if ((e = hipMalloc(&c->U, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U");
   if ((e = hipMalloc(&c->U1, bytes * c->nvt)) != hipSuccess) return cleanup("hipMalloc U1");
@@ -891,20 +921,57 @@ int pip_adj_seed(pip_ctx* c, double* lam) {
   return PIP_OK;
 }
 
+// One adjoint step with the face-state work buffer P and the stage-1
+// adjoint lamB given.
+static int adj_step_with(pip_ctx* c, const double* Un, const double* U1, const double* Vn, const double* V1,
+                         const double* dtc, double* lam, double* lam_ac, double* P, double* lamB) {
+  // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1):
+  // lam -> lamB (adjoint of U1)
+  AdjArgs a2 = adj_args(c, 2, U1, V1, lam, lamB, nullptr, lam_ac, dtc);
+  a2.P = P;
+  TRY(adj_stage(c, a2));
+  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB and the c0 part of
+  // lam (the U0 path of stage 2) -> lam
+  AdjArgs a1 = adj_args(c, 1, Un, Vn, lamB, lam, lam, lam_ac, dtc);
+  a1.P = P;
+  TRY(adj_stage(c, a1));
+  return PIP_OK;
+}
+
 int pip_adj_step(pip_ctx* c, const double* Un, const double* U1, const double* Vn, const double* V1,
                  const double* dtc, double* lam, double* lam_ac) {
   if (!c || !Un || !U1 || !dtc || !lam) return fail(PIP_ERR_ARG, "null argument");
   if (!Vn != !V1) return fail(PIP_ERR_ARG, "give both saved primitive states or neither");
   TRY(adj_check(c));
   TRY(adj_alloc(c));
-  // stage 2: U_{n+1} = c0 U_n + c1 U1 - dt_sub div F(U1) + dt_sub S(U1):
-  // lam -> lamB (adjoint of U1)
-  TRY(adj_stage(c, adj_args(c, 2, U1, V1, lam, c->lamB, nullptr, lam_ac, dtc)));
-  // stage 1: U1 = U_n - dt div F(U_n) + dt S(U_n): lamB and the c0 part of
-  // lam (the U0 path of stage 2) -> lam
-  AdjArgs a1 = adj_args(c, 1, Un, Vn, c->lamB, lam, lam, lam_ac, dtc);
-  TRY(adj_stage(c, a1));
-  return PIP_OK;
+  return adj_step_with(c, Un, U1, Vn, V1, dtc, lam, lam_ac, c->adjP, c->lamB);
+}
+
+int pip_adj_step_noalloc(pip_ctx* c, const double* Un, const double* U1, const double* Vn, const double* V1,
+                         const double* dtc, double* lam, double* lam_ac) {
+  if (!c || !Un || !U1 || !Vn || !V1 || !dtc || !lam)
+    return fail(PIP_ERR_ARG, "pip_adj_step_noalloc: null argument (Vn and V1 are required)");
+  TRY(adj_check(c));
+  const size_t need = sizeof(double) * (size_t)(5 * c->nvt + 4) * (size_t)c->N;
+  if (need > kAdjScratchBytes)
+    return fail(PIP_ERR_ARG, "pip_adj_step_noalloc: the grid needs " + std::to_string(need) +
+                                 " bytes of work memory, more than its " + std::to_string(kAdjScratchBytes) +
+                                 "-byte device array; use pip_adj_step");
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_done[c->device])
+      return fail(PIP_ERR_STATE, "pip_adj_step_noalloc: no work array on this device");
+    if (!g_scratch_ptr[c->device])
+      return fail(PIP_ERR_HIP, std::string("pip_adj_step_noalloc: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(g_scratch_err[c->device]));
+    base = g_scratch_ptr[c->device];
+  }
+  // P: 2 directions x nvt planes of (left, right) pairs, then 2 directions
+  // of code pairs; lamB after it
+  double* P = base;
+  double* lamB = base + (size_t)(4 * c->nvt + 4) * (size_t)c->N;
+  return adj_step_with(c, Un, U1, Vn, V1, dtc, lam, lam_ac, P, lamB);
 }
 
 }  // extern "C"
