@@ -156,16 +156,31 @@ def read(indir: str, nout: int, cfg: PIPConfig, ac: bool = False, rates: bool = 
     return {k: v for k, v in out.items() if v is not None}
 
 
+def output_fields(cfg: PIPConfig, ac, rates=None, xi=None) -> dict:
+    """The coefficient planes a two-fluid output hands to ``write``: ac and
+    xi always (def_varfiles writes both for flag_pip = 1, IO_rot.f90:234-237),
+    and with ionisation/recombination the rates gm_ion, gm_rec and, for
+    flag_IR_type = 0, arb_heat (IO_rot.f90:238-241, 254-256, 303-306).
+    rates = (gm_rec, gm_ion, arb_heat).  Runs without ambipolar diffusion
+    never set xi_n (set_xin runs only for flag_amb >= 1,
+    Solver_rot.f90:108-110), so the reference writes an unset allocation
+    there; the default here is a plane of zeros."""
+    xi = np.zeros((cfg.jx, cfg.ix)) if xi is None else xi
+    out = dict(ac=ac, xi=xi)
+    if cfg.flag_ir:
+        gm_rec, gm_ion, arb = rates
+        out.update(gm_ion=gm_ion, gm_rec=gm_rec, arb_heat=arb if cfg.flag_ir_type == 0 else None)
+    return out
+
+
 def write_solver(solver, outdir: str, nout: int, xi=None, **sav) -> str:
-    """Output of a PIPSolver's current state (downloaded), time, ac and, with
-    ionisation/recombination, the rate fields."""
+    """Output of a PIPSolver's current state (downloaded), time and the
+    coefficient planes of ``output_fields``."""
     cfg = solver.cfg
     U_h, U_m = solver.state()
-    ac = solver.ac_field()
-    rates = solver.rates() if cfg.flag_ir else (None, None, None)
-    gm_rec, gm_ion, arb = rates
-    return write(outdir, nout, cfg, U_h, U_m, solver.scalars()["time"], ac=ac, xi=xi, gm_ion=gm_ion,
-                 gm_rec=gm_rec, arb_heat=arb if cfg.flag_ir_type == 0 else None, **sav)
+    rates = solver.rates() if cfg.flag_ir else None
+    return write(outdir, nout, cfg, U_h, U_m, solver.scalars()["time"],
+                 **output_fields(cfg, solver.ac_field(), rates, xi), **sav)
 
 
 def restart_solver(solver, indir: str, nout: int):

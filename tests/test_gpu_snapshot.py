@@ -11,6 +11,7 @@ import pytest
 from pip_amd import build, problems, snapshot
 from pip_amd.config import PIPConfig
 from pip_amd.solver import PIPSolver
+from tests.util import h5_listing, snapshot_case
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(os.path.join(build.HDF5_PREFIX, "include", "hdf5.h")),
@@ -29,6 +30,11 @@ def test_output_restart_continue(tmp_path, case):
     t6 = s.scalars()["time"]
     path = snapshot.write_solver(s, str(tmp_path), 1)
     assert os.path.exists(path)
+    # the datasets the reference's output path creates for these flags
+    ls = h5_listing(path)
+    if ls is not None:
+        want = snapshot_case("ir_3_type_0_col_3_later" if cfg.flag_ir else "orszag_tang_later_output")
+        assert set(ls) == set(want["datasets"])
     snap = snapshot.read(str(tmp_path), 1, cfg, ac=True, rates=bool(cfg.flag_ir))
     h6, m6 = s.state()
     assert np.array_equal(snap["U_h"], h6) and np.array_equal(snap["U_m"], m6) and snap["time"] == t6

@@ -10,17 +10,13 @@ taken with the HDF5 tools' h5dump -H; the round trip through the restart
 reader is bitwise.
 """
 import os
-import re
-import shutil
-import subprocess
 
 import numpy as np
 import pytest
 
 from pip_amd import build, problems, snapshot
 from pip_amd.config import MARGIN, PIPConfig
-
-H5DUMP = shutil.which("h5dump") or os.path.join(build.HDF5_PREFIX, "bin", "h5dump")
+from tests.util import SNAPSHOT_TABLE, h5_listing
 
 pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(build.HDF5_PREFIX, "include", "hdf5.h")),
                                 reason="no HDF5 C library in this image")
@@ -34,15 +30,9 @@ def _lib():
 
 
 def _listing(path):
-    """{dataset name: (dims tuple or 'SCALAR', type)} from h5dump -H."""
-    if not os.path.exists(H5DUMP):
+    out = h5_listing(path)
+    if out is None:
         pytest.skip("h5dump not installed")
-    txt = subprocess.run([H5DUMP, "-H", path], capture_output=True, text=True, check=True).stdout
-    out = {}
-    for m in re.finditer(r'DATASET "([^"]+)" \{\s*DATATYPE\s+(\S+)\s*DATASPACE\s+(SCALAR|SIMPLE \{ \( ([^)]*) \))',
-                         txt):
-        dims = "SCALAR" if m.group(3) == "SCALAR" else tuple(int(v) for v in m.group(4).split(","))
-        out[m.group(1)] = (dims, m.group(2))
     return out
 
 
@@ -130,3 +120,26 @@ def test_reader_errors(tmp_path):
         snapshot.read(str(tmp_path), 5, other)
     with pytest.raises(snapshot.PipioError, match="nout"):
         snapshot.write(str(tmp_path), 10000, cfg, uh, um, 0.5)
+
+
+@pytest.mark.parametrize("case", SNAPSHOT_TABLE["cases"], ids=[c["name"] for c in SNAPSHOT_TABLE["cases"]])
+def test_dataset_table_matches_the_reference_output_routine(tmp_path, case):
+    """The datasets of one output, written through output_fields (the
+    selection write_solver uses) with the case's flags and *_sav keys, are
+    those the reference's output path creates (golden/snapshot_datasets.json,
+    walked out of IO_rot.f90 by golden/make_snapshot_table.py); a name the
+    reference creates twice is one dataset."""
+    f = dict(case["flags"])
+    base = problems.brio_wu(nx=16) if f["nvar_m"] == 8 else problems.orszag_tang(12)
+    cfg, uh, um, _ = base
+    cfg = PIPConfig(**{**cfg.__dict__, **{k: f[k] for k in ("flag_divb", "flag_col", "flag_ir", "flag_ir_type")}})
+    assert cfg.nvar_m == f["nvar_m"]
+    plane = np.ones((cfg.jx, cfg.ix))
+    rates = (plane * 2, plane * 3, plane * 4) if cfg.flag_ir else None
+    fields = snapshot.output_fields(cfg, plane, rates)
+    sav = {k: f[k] for k in ("ac_sav", "xi_sav", "col_sav", "ion_sav", "rec_sav", "heat_sav", "ps_sav")}
+    path = snapshot.write(str(tmp_path), case["nout"], cfg, uh, um, 0.5, **fields, **sav)
+    ls = _listing(path)
+    assert set(ls) == set(case["datasets"])
+    for n in set(case["datasets"]) - set(COORDS + ["time"]):
+        assert ls[n] == ((1, cfg.jx, cfg.ix), "H5T_IEEE_F64LE"), n
