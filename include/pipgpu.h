@@ -84,6 +84,10 @@ int pip_bnd(pip_ctx *ctx);
  * time >= tend, max_steps steps, or a stop_sim condition.  steps_done and
  * time may be NULL. */
 int pip_advance(pip_ctx *ctx, int max_steps, double tend, int *steps_done, double *time);
+/* 1 (default): pip_advance replays its 64-step chunks from a hipGraph
+ * captured on first use (one graph launch instead of 3 launches per step;
+ * the same kernels, so the same results); 0: plain stream launches. */
+int pip_set_step_graph(pip_ctx *ctx, int on);
 /* Enqueue n fused steps without waiting (no host sync); for timing loops. */
 int pip_enqueue_steps(pip_ctx *ctx, int n, double tend);
 int pip_sync(pip_ctx *ctx);
@@ -117,6 +121,24 @@ int pip_set_tile(pip_ctx *ctx, int tx, int ty);
  * constant or per-cell alpha, other runs keep the one-pass kernel); clear:
  * both fluids at once.  Bitwise the same results (autotuner). */
 int pip_set_two_pass(pip_ctx *ctx, int stage_mask);
+/* Arithmetic of the stage kernels.  PIP_ARITH_EXACT (the default): the
+ * reference's operations in the reference's order, no FMA contraction,
+ * correctly rounded quotients and square roots -- bitwise the CPU reference
+ * path.  PIP_ARITH_RELAXED: the same operations with FMA contraction,
+ * quotients as products with a refined reciprocal and the hardware min/max.
+ * Not bitwise: after 100 smooth steps the variables of order one
+ * (densities, energies, the background field, a shear flow's momentum) stay
+ * within 1e-10 relative L1 of the reference path, but quantities set by a
+ * small perturbation or by round-off (transverse momenta and fields, psi)
+ * differ by up to ~1e-5 of their own size; shocks stay at the same cells.
+ * Relaxed covers 2-D runs with psi (with or without set_IR, tiles 16x15,
+ * 32x7, 16x7) and 1-D runs without set_IR, with constant or per-cell
+ * alpha_c and explicit collisions (PIP_ERR_UNSUPPORTED otherwise); it
+ * ignores the two-pass stage mask, and the adjoint's frozen-dt replays stay
+ * exact. */
+#define PIP_ARITH_EXACT 0
+#define PIP_ARITH_RELAXED 1
+int pip_set_arith(pip_ctx *ctx, int mode);
 /* 1: map workgroups to tiles so that each of the 8 XCDs gets a contiguous
  * range of tiles (neighbouring halos in one L2); 0: row-major (autotuner). */
 int pip_set_xcd_remap(pip_ctx *ctx, int on);
@@ -175,10 +197,10 @@ int pip_adj_step_noalloc(pip_ctx *ctx, const double *Un, const double *U1, const
                          const double *V1, const double *dtc, double *lam, double *lam_ac);
 
 /* ---- batched ensemble (no counterpart in the reference: PIP runs one
- * domain per process).  Independent 1-D two-fluid runs of different sizes,
- * each with its own dt/time/step count, integrated in ONE persistent launch
- * (optionally replayed from a hipGraph); every member equals its plain
- * pip_advance run bit for bit. */
+ * domain per process).  Independent 1-D or 2-D two-fluid runs of different
+ * sizes, each with its own dt/time/step count, integrated together (one
+ * persistent launch, lockstep launches, or lockstep chunks replayed from a
+ * hipGraph); every member equals its plain pip_advance run bit for bit. */
 typedef struct pip_ens_member {
     int nx;            /* interior cells (ix = nx + 4)                     */
     double dx;         /* cell width                                      */
@@ -186,9 +208,14 @@ typedef struct pip_ens_member {
     double tend;       /* end time of this run (main.f90:31)               */
     int max_steps;     /* step cap                                        */
     double cost;       /* expected work, for longest-first scheduling     */
+    int ny;            /* 2-D ensembles: interior rows (jx = ny + 4)       */
+    double dy;         /* 2-D ensembles: cell height                       */
 } pip_ens_member;
 typedef struct pip_ens pip_ens;
-/* base supplies gm, safety, limits, flag_bnd, db_clean; ndim must be 1. */
+/* base supplies gm, safety, limits, flag_bnd, db_clean and the dimension:
+ * ndim = 1 (nvar_m = 8; members nx, dx) or ndim = 2 with psi (nvar_m = 9,
+ * flag_divb = 1; members nx, ny, dx, dy; lockstep modes only).  Every member
+ * has the layout of a plain context: nvar_m + 5 planes of ix * jx. */
 int pip_ens_create(const pip_cfg *base, const pip_ens_member *members, int n, pip_ens **out);
 int pip_ens_upload(pip_ens *e, int m, const double *U_h, const double *U_m);
 int pip_ens_download(pip_ens *e, int m, double *U_h, double *U_m);
@@ -197,9 +224,12 @@ int pip_ens_set_tile(pip_ens *e, int tx);
 /* 0: one persistent launch, a workgroup per member (default; use_graph
  * replays it from a hipGraph); 1: lockstep, each member's tiles spread over
  * workgroups, two stage launches per step for all running members (stage 1
- * applies the previous step's end), host-driven in 64-step chunks
- * (use_graph ignored).  Both give every member's plain pip_advance run bit
- * for bit. */
+ * applies the previous step's end), host-driven in 64-step chunks with the
+ * tile table rebuilt between chunks (use_graph ignored); 2: graph-captured
+ * lockstep, one 64-step chunk over the fixed table of every member's tiles
+ * captured once as a hipGraph (stopped members' tiles return at once) and
+ * replayed until no member runs, no table rebuild (use_graph ignored).  All
+ * give every member's plain pip_advance run bit for bit. */
 int pip_ens_set_mode(pip_ens *e, int mode);
 /* restore every member to its uploaded state (device copy, not timed) */
 int pip_ens_reset(pip_ens *e);

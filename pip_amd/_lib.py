@@ -49,7 +49,8 @@ def cfg_to_c(cfg) -> PipCfg:
 class PipEnsMember(ctypes.Structure):
     """pip_ens_member of include/pipgpu.h."""
     _fields_ = [("nx", ctypes.c_int), ("dx", ctypes.c_double), ("col", ctypes.c_double),
-                ("tend", ctypes.c_double), ("max_steps", ctypes.c_int), ("cost", ctypes.c_double)]
+                ("tend", ctypes.c_double), ("max_steps", ctypes.c_int), ("cost", ctypes.c_double),
+                ("ny", ctypes.c_int), ("dy", ctypes.c_double)]
 
 
 class PipError(RuntimeError):
@@ -90,6 +91,8 @@ _SIGS = {
     "pip_set_tile": [_V, _I, _I],
     "pip_set_xcd_remap": [_V, _I],
     "pip_set_two_pass": [_V, _I],
+    "pip_set_arith": [_V, _I],
+    "pip_set_step_graph": [_V, _I],
     "pip_time_steps": [_V, _I, _PD, _PD, _PD],
     "pip_set_state_device": [_V, _V],
     "pip_get_state_device": [_V, _V],

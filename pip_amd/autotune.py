@@ -14,8 +14,11 @@ CACHE = os.path.join(build.LIBDIR, "autotune.json")
 STAGE_TILES = [(16, 15), (32, 7), (32, 15), (64, 7), (32, 8), (16, 7)]
 XCD_REMAP = [0, 1]  # row-major tile order, or contiguous tile ranges per XCD
 TWO_PASS_TILES = [(16, 15), (32, 7), (16, 7)]  # tiles of the fluid-by-fluid stage kernel
+RELAXED_TILES = [(16, 15), (32, 7), (16, 7)]  # tiles of the relaxed-arithmetic stage kernel
 ENS_TILES = [64, 128, 256, 512]
-ENS_MODES = ["persistent", "lockstep"]  # one launch, or every member spread over workgroups
+# one launch; every member spread over workgroups in host-driven chunks; the
+# same chunks captured as a hipGraph
+ENS_MODES = ["persistent", "lockstep", "graph"]
 
 
 def _load():
@@ -32,9 +35,10 @@ def _save(d):
     json.dump(d, open(CACHE, "w"), indent=1)
 
 
-def stage_key(cfg) -> str:
-    return (f"stage2:{cfg.ndim}:{cfg.nx}x{cfg.ny}:nv{cfg.nvar_m}:ir{cfg.flag_ir}:ac{int(cfg.ac_field)}"
-            f":col{cfg.flag_col}:imp{cfg.flag_pip_imp}:twopass")
+def stage_key(cfg, arith: str = "exact") -> str:
+    key = (f"stage2:{cfg.ndim}:{cfg.nx}x{cfg.ny}:nv{cfg.nvar_m}:ir{cfg.flag_ir}:ac{int(cfg.ac_field)}"
+           f":col{cfg.flag_col}:imp{cfg.flag_pip_imp}:twopass")
+    return key if arith == "exact" else key + ":" + arith
 
 
 def two_pass_ok(cfg) -> bool:
@@ -42,17 +46,22 @@ def two_pass_ok(cfg) -> bool:
     return cfg.ndim == 2 and cfg.flag_ir == 0 and cfg.flag_col == 1 and cfg.flag_pip_imp == 0
 
 
-def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True):
+def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True, arith: str = "exact"):
     """make_solver((tx, ty, xcd_remap, two_pass)) -> PIPSolver ready to step
     (two_pass: mask of the SSP-RK stages that take the fluids one at a
-    time).  Returns (best (tx, ty, xcd_remap, two_pass), timings)."""
+    time) in the given arithmetic (the relaxed kernels come in three tiles,
+    one pass).  Returns (best (tx, ty, xcd_remap, two_pass), timings)."""
     cache = _load()
-    key = stage_key(cfg)
+    key = stage_key(cfg, arith)
     if use_cache and key in cache:
         return tuple(cache[key]["tile"]), cache[key]["ms"]
     timings = {}
-    cands = [(t[0], t[1], r, 0) for t in STAGE_TILES for r in XCD_REMAP]
-    if two_pass_ok(cfg):
+    if arith == "relaxed":
+        # 1-D runs have one kernel (255-cell tiles); the tile setting is unused there
+        cands = [(t[0], t[1], r, 0) for t in RELAXED_TILES for r in XCD_REMAP] if cfg.ndim == 2 else [(16, 15, 0, 0)]
+    else:
+        cands = [(t[0], t[1], r, 0) for t in STAGE_TILES for r in XCD_REMAP]
+    if arith == "exact" and two_pass_ok(cfg):
         full = (1 << cfg.t_order) - 1
         cands += [(t[0], t[1], r, m) for t in TWO_PASS_TILES for r in XCD_REMAP
                   for m in sorted({1 << (cfg.t_order - 1), full})]
@@ -85,6 +94,9 @@ def tune_ensemble(ens, key: str, use_cache: bool = True):
         ens.set_mode(mode)
         for tx in ENS_TILES:
             ens.set_tile(tx)
+            if mode == "graph":  # the first run captures the graph
+                ens.reset()
+                ens.run(graph=False)
             ens.reset()
             timings[f"{mode}:{tx}"] = ens.run(graph=False)
     best = min(timings, key=timings.get)

@@ -7,7 +7,7 @@
 
 #include "pip_phys.h"
 
-namespace pip {
+namespace PIP_NS {
 
 // Largest padded extent (ix or jx) the packed image list can address.
 constexpr int kMaxPad = 32767;
@@ -151,4 +151,4 @@ PIP_HD void for_each_image(int ci, int cj, int ix, int jx, const Bnd& b, Fn fn) 
   }
 }
 
-}  // namespace pip
+}  // namespace PIP_NS

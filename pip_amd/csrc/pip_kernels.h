@@ -22,7 +22,7 @@
 #include "pip_bnd.h"
 #include "pip_phys.h"
 
-namespace pip {
+namespace PIP_NS {
 
 // Reduction slots (bit patterns of non-negative doubles; min/max as uint64).
 enum { RED_DT = 0, RED_RHOAC = 1, RED_GMAX = 2, RED_IONPOT = 3, RED_NAN = 4, RED_N = 5 };
@@ -113,7 +113,7 @@ __device__ __forceinline__ RoundOut cfl_round(double* um, double* uh, double ac,
   if (IR && a.ph.flag_ir_type == 0) {
     double pr, te;
     getpr_mhd(um, pr, te, a.ph);  // cfl_pip_ir :321
-    r.ratio = ion_pot / (pr / (a.ph.gm - 1.0));
+    r.ratio = fdiv(ion_pot, fdiv(pr, a.ph.gm - 1.0));
   }
   return r;
 }
@@ -829,9 +829,9 @@ __global__ void k_finalize(FinArgs f) {
   }
 }
 
-}  // namespace pip
+}  // namespace PIP_NS
 
-namespace pip {
+namespace PIP_NS {
 // Small control kernel: set loop bookkeeping fields on the device.
 __global__ void k_set_loop(DevScalars* sc, double tend, long long nmax, int active, int ion_valid) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -840,9 +840,9 @@ __global__ void k_set_loop(DevScalars* sc, double tend, long long nmax, int acti
   if (active >= 0) sc->active = active;
   if (ion_valid >= 0) sc->ion_pot_valid = ion_valid;
 }
-}  // namespace pip
+}  // namespace PIP_NS
 
-namespace pip {
+namespace PIP_NS {
 // ------------------------------------------------------ batched ensemble
 // One persistent launch integrates an ensemble of independent 1-D runs of
 // different sizes: each workgroup pulls a member index from a device queue
@@ -851,10 +851,13 @@ namespace pip {
 // dt, time and step count.  The arithmetic per cell is that of the plain
 // path, so every member equals its plain run bit for bit.
 struct EnsMember {
-  long long off;  // offset (doubles) of the member's block: 13 planes of ix
+  long long off;  // offset (doubles) of the member's block: nv + 5 planes of ix * jx
   int nx, ix;
   double dx, col, gmin_safety, tend;
   long long nmax;
+  // 2-D members (ny = 0 in 1-D): rows, padded rows, dy, min(dx, dy, dz)
+  int ny, jx;
+  double dy, dxyz_min;
 };
 struct EnsOut {
   double time, dt;
@@ -1075,7 +1078,7 @@ __global__ void __launch_bounds__(TX) k_ensemble(EnsArgs e) {
 // bit; the members' tiles run in parallel, as in the plain path, instead of
 // one after another on one CU.
 struct LockTile {
-  int m, i0;  // member, first cell of the tile
+  int m, i0, j0, pad;  // member, first cell of the tile (column, row)
 };
 
 // The persistent loop's step end for one member: dt of the next step from
@@ -1104,6 +1107,7 @@ __device__ __forceinline__ DevScalars lock_step_end(const DevScalars& sc, const 
   o.status = status;
   if (o.time < M.tend && o.nt < M.nmax && status == 0) {
     o.dt = dt_new;
+    if (M.ny > 0) o.cmax = safety * M.dxyz_min / dt_new;  // GLM speed of 2-D members (psi)
     o.last = ((o.time + dt_new >= M.tend) || (o.nt + 1 >= M.nmax) || (dt_new <= 1.0e-12)) ? 1 : 0;
   } else {
     o.active = 0;
@@ -1194,6 +1198,159 @@ __global__ void __launch_bounds__(TX) k_lock_stage(EnsArgs e, DevScalars* scs, c
   }
 }
 
+// Number of members still running, from scalar buffer 0 (where a chunk's
+// closing k_lock_end leaves them); one workgroup.
+__global__ void k_lock_count(const DevScalars* scs, int n, int* active) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int c = 0;
+  for (int m = (int)threadIdx.x; m < n; m += (int)blockDim.x) c += scs[m].active ? 1 : 0;
+  if (c) atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) *active = cnt;
+}
+
+// ---------------------------------------------------- 2-D ensemble members
+// Members of a 2-D ensemble: the plain path's 2-D stage (psi, no IR,
+// constant alpha_c, explicit collisions) on 16x15 tiles of 256 threads,
+// every member's tiles side by side in the lockstep launches.
+constexpr int kEns2TX = 16, kEns2TY = 15, kEns2NT = 256;
+
+__device__ __forceinline__ StageArgs ens_stage_args2(const EnsArgs& e, const EnsMember& M) {
+  StageArgs a = ens_stage_args(e, M);
+  a.jx = M.jx;
+  a.ny = M.ny;
+  a.N = (long long)M.ix * M.jx;
+  a.dy = M.dy;
+  a.dxyz_min = M.dxyz_min;
+  return a;
+}
+
+// cfl rewrite round over all cells of a 2-D member (ghosts included), as
+// k_round does for the plain path; returns the reduced dt terms.
+template <int NT>
+__device__ __forceinline__ RoundOut ens_cfl_cells2(double* U, const StageArgs& a, double* lds) {
+  RoundOut acc = {1.0e300, 0.0, 0.0, 0.0};
+  int fl = 0;
+  for (long long k = threadIdx.x; k < a.N; k += NT) {
+    double u[14];
+#pragma unroll
+    for (int n = 0; n < 14; n++) u[n] = U[n * a.N + k];
+    const double dtc = cfl_cell<9>(u, u + 9, a.gmin_safety, a.ph);
+    const int i = (int)(k % a.ix), j = (int)(k / a.ix);
+    if (i >= 2 && i < a.ix - 2 && j >= 2 && j < a.jx - 2) acc.dtc = dmin(acc.dtc, dtc);
+    acc.rhoac = dmax(acc.rhoac, dmax(u[0] * a.col, u[9] * a.col));
+#pragma unroll
+    for (int n = 0; n < 14; n++) U[n * a.N + k] = u[n];
+  }
+  return wg_reduce<NT>(acc, fl, lds);
+}
+
+// Start of a 2-D member by one workgroup: the boundary map of the initial
+// state (ghosts from interior sources, k_bnd's composite map), the prologue
+// cfl and the loop-top cfl; writes both scalar buffers.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_lock_start2(EnsArgs e, DevScalars* scs) {
+  __shared__ double lds_red[3 * (NT / 64)];
+  const int m = blockIdx.x;
+  if (m >= e.n_members) return;
+  const EnsMember M = e.mem[m];
+  const StageArgs a = ens_stage_args2(e, M);
+  double* U = e.U + M.off;
+  for (long long k = threadIdx.x; k < a.N; k += NT) {
+    const int i = (int)(k % a.ix), j = (int)(k / a.ix);
+    const bool gx = (i < 2 || i >= a.ix - 2), gy = (j < 2 || j >= a.jx - 2);
+    if (!gx && !gy) continue;
+    bool fx = false, fy = false;
+    const int si = gx ? src_index(i, a.ix, e.bnd.type[0], e.bnd.type[1], fx) : i;
+    const int sj = gy ? src_index(j, a.jx, e.bnd.type[2], e.bnd.type[3], fy) : j;
+    unsigned msk = 0u;
+    if (fx) msk ^= e.bnd.sym[(i < 2) ? 0 : 1];
+    if (fy) msk ^= e.bnd.sym[(j < 2) ? 2 : 3];
+    const long long src = (long long)sj * a.ix + si;
+    for (int n = 0; n < 14; n++) {
+      const double v = U[n * a.N + src];
+      U[n * a.N + k] = ((msk >> n) & 1u) ? -v : v;
+    }
+  }
+  __syncthreads();
+  ens_cfl_cells2<NT>(U, a, lds_red);
+  __syncthreads();
+  const RoundOut o = ens_cfl_cells2<NT>(U, a, lds_red);
+  const double dt = ens_dt(o, e.ph.safety);
+  if (threadIdx.x == 0) {
+    DevScalars sc;
+    sc.dt = dt;
+    sc.time = 0.0;
+    sc.cmax = e.ph.safety * M.dxyz_min / dt;
+    sc.tend = M.tend;
+    sc.dt_out = dt;
+    sc.nt = 0;
+    sc.nmax = M.nmax;
+    sc.status = 0;
+    sc.ion_pot_valid = 0;
+    sc.red[RED_DT] = as_u64(1.0e300);
+    sc.red[RED_RHOAC] = 0ull;
+    sc.red[RED_GMAX] = 0ull;
+    sc.red[RED_IONPOT] = 0ull;
+    sc.red[RED_NAN] = 0ull;
+    const bool go = (0.0 < M.tend) && (0 < M.nmax);
+    sc.active = go ? 1 : 0;
+    sc.last = ((0.0 + dt >= M.tend) || (1 >= M.nmax) || (dt <= 1.0e-12)) ? 1 : 0;
+    scs[m] = sc;
+    scs[e.n_members + m] = sc;
+    if (!go) e.out[m] = EnsOut{0.0, dt, 0, 0, 0};
+  }
+}
+
+// Stage ISTEP of step k (p = k & 1) for one 16x15 tile of a 2-D member, as
+// k_lock_stage does for 1-D members.
+template <int ISTEP>
+__global__ void __launch_bounds__(kEns2NT, 2) k_lock_stage2(EnsArgs e, DevScalars* scs, const LockTile* tiles,
+                                                            int p, int apply_end) {
+  extern __shared__ double smem[];
+  const LockTile t = tiles[blockIdx.x];
+  const EnsMember M = e.mem[t.m];
+  DevScalars* sc = scs + (long long)p * e.n_members + t.m;
+  double dt, cmax;
+  int last;
+  if (ISTEP == 1 && apply_end) {
+    bool stops;
+    const DevScalars nw = lock_step_end(scs[(long long)(p ^ 1) * e.n_members + t.m], M, e.ph.safety, &stops);
+    if (t.i0 == 0 && t.j0 == 0 && threadIdx.x == 0) {
+      *sc = nw;
+      if (stops) e.out[t.m] = EnsOut{nw.time, nw.dt, nw.nt, nw.status, 0};
+    }
+    if (!nw.active) return;
+    dt = nw.dt;
+    cmax = nw.cmax;
+    last = nw.last;
+  } else {
+    if (!sc->active) return;
+    dt = sc->dt;
+    cmax = sc->cmax;
+    last = sc->last;
+  }
+  StageArgs a = ens_stage_args2(e, M);
+  double* U = e.U + M.off;
+  double* U1 = e.U1 + M.off;
+  a.istep = ISTEP;
+  a.Uin = (ISTEP == 1) ? U : U1;
+  a.U0 = U;
+  a.Uout = (ISTEP == 1) ? U1 : U;
+  a.epi = 0;
+  const int epi = (ISTEP == 1) ? 0 : (last ? 1 : 2);
+  RoundOut r;
+  int nanflag;
+  const bool valid = stage_tile<2, 9, false, false, kEns2TX, kEns2TY, kEns2NT>(a, dt, cmax, epi, t.i0, t.j0, smem,
+                                                                               r, nanflag);
+  if (ISTEP == 2) {
+    __syncthreads();
+    block_reduce_store<kEns2NT>(r, nanflag, valid, sc, smem);
+  }
+}
+
 // Step end of the listed members (those of the chunk's table) after step k,
 // p = k & 1: from buffer p into buffer p ^ 1; a member that stops writes its
 // EnsOut.
@@ -1206,4 +1363,4 @@ __global__ void k_lock_end(EnsArgs e, DevScalars* scs, const int* members, int c
   scs[(long long)(p ^ 1) * e.n_members + m] = nw;
   if (stops) e.out[m] = EnsOut{nw.time, nw.dt, nw.nt, nw.status, 0};
 }
-}  // namespace pip
+}  // namespace PIP_NS

@@ -17,7 +17,23 @@
 
 #include <math.h>
 
-namespace pip {
+// PIP_NS: the namespace of everything below (pip; a second build of the
+// same headers uses another, so that both sets of kernels live in one
+// library).  PIP_RELAXED = 1 selects the relaxed device arithmetic of that
+// build (see "Relaxed arithmetic" below); 0, the default, the exact one.
+#ifndef PIP_NS
+#define PIP_NS pip
+#endif
+#ifndef PIP_RELAXED
+#define PIP_RELAXED 0
+#endif
+#if PIP_RELAXED && defined(__HIP_DEVICE_COMPILE__)
+#define PIP_RELAXED_DEVICE 1
+#else
+#define PIP_RELAXED_DEVICE 0
+#endif
+
+namespace PIP_NS {
 
 constexpr double kTiny = 1.0e-20;  // Globalvar.f90:101
 constexpr double kEpsi = 1.0e-10;  // HLL_rot.f90:12
@@ -26,8 +42,20 @@ constexpr double kHllg = 1.001;    // HLL_rot.f90:701
 constexpr double kMuP = 0.5;       // PIP_rot.f90:36
 constexpr double kMuN = 1.0;       // PIP_rot.f90:37
 
+// MIN / MAX of the reference.  The device uses the hardware instructions
+// (v_min_f64 / v_max_f64: one instruction instead of a compare and two
+// selects).  For operands that are not NaN they return the same value as
+// the compare-and-select of the host build, except which zero comes back
+// for min(-0, +0) and max(+0, -0); on this path such zeros only arise in
+// variables that are identically zero (B_z, m_z of a planar flow), where
+// no later operation distinguishes them, so states compare equal.
+#if defined(__HIP_DEVICE_COMPILE__)
+PIP_HD double dmin(double a, double b) { return fmin(a, b); }
+PIP_HD double dmax(double a, double b) { return fmax(a, b); }
+#else
 PIP_HD double dmin(double a, double b) { return (a < b) ? a : b; }
 PIP_HD double dmax(double a, double b) { return (a > b) ? a : b; }
+#endif
 PIP_HD double sgl(double x) { return (double)(float)x; }  // Fortran default-real literal
 
 // ------------------------------------------------------------ quotients
@@ -47,10 +75,16 @@ __device__ __forceinline__ double qrcp(double d) {
   r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
   return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
 }
+#if PIP_RELAXED
+// relaxed: the product with the refined reciprocal (within an ulp or two of
+// the quotient; no residual correction, no fixup)
+__device__ __forceinline__ double quo(double n, double, double r) { return n * r; }
+#else
 __device__ __forceinline__ double quo(double n, double d, double r) {
   const double q = n * r;
   return __builtin_amdgcn_div_fixup(__builtin_fma(r, __builtin_fma(-d, q, n), q), d, n);
 }
+#endif
 // qsqrt(x) is sqrt(x) the same way: the steps of the general double square
 // root sequence -- v_rsq_f64 y, g = x y, h = y/2, r = 1/2 - h g, g += g r,
 // h += h r, two corrections g += h (x - g g), x itself for +-0 and +inf --
@@ -71,6 +105,18 @@ PIP_HD double quo(double n, double d, double) { return n / d; }
 PIP_HD double qsqrt(double x) { return sqrt(x); }
 #endif
 PIP_HD double qdiv(double n, double d) { return quo(n, d, qrcp(d)); }
+
+// fdiv(n, d) and fsqrt(x): the quotients and square roots of the set_IR,
+// get_Pr and cfl terms.  Exact build: the IEEE operations (general sequences,
+// valid over the whole range, as the rates can underflow); relaxed device
+// build: the refined reciprocal times n and qsqrt.
+#if PIP_RELAXED_DEVICE
+PIP_HD double fdiv(double n, double d) { return n * qrcp(d); }
+PIP_HD double fsqrt(double x) { return qsqrt(x); }
+#else
+PIP_HD double fdiv(double n, double d) { return n / d; }
+PIP_HD double fsqrt(double x) { return sqrt(x); }
+#endif
 
 // ------------------------------------------------------ exp / log / pow
 // exp, log and pow as fixed sequences of IEEE double additions and
@@ -116,7 +162,7 @@ PIP_HD double plog(double x) {
     m = m * 2.0;
     e = e - 1;
   }
-  double s = (m - 1.0) / (m + 1.0);
+  double s = fdiv(m - 1.0, m + 1.0);
   double s2 = s * s;
   double t = 1.0 / 25.0;
   t = t * s2 + 1.0 / 23.0;
@@ -249,20 +295,20 @@ PIP_HD void cq2pv_hd(double* u, double* prim, const Phys& p) {
 // get_Pr_HD (+get_Te_HD), rewrites u[4]: Scheme_rot.f90:83-102
 PIP_HD void getpr_hd(double* u, double& pr, double& te, const Phys& p) {
   double s = (u[1] * u[1] + u[2] * u[2]) + u[3] * u[3];
-  pr = (p.gm - 1.0) * (u[4] - (0.5 * s) / u[0]);
+  pr = (p.gm - 1.0) * (u[4] - fdiv(0.5 * s, u[0]));
   pr = dmax(pr, p.pr_lim);
-  u[4] = pr / (p.gm - 1.0) + 0.5 * (s / u[0]);
-  te = p.gm * (pr / u[0]);
+  u[4] = fdiv(pr, p.gm - 1.0) + 0.5 * fdiv(s, u[0]);
+  te = p.gm * fdiv(pr, u[0]);
 }
 
 // get_Pr_MHD (+get_Te_MHD), rewrites u[4]: Scheme_rot.f90:104-128
 PIP_HD void getpr_mhd(double* u, double& pr, double& te, const Phys& p) {
-  double q = ((((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3]) / u[0] + u[5] * u[5]) + u[6] * u[6]) +
+  double q = ((fdiv((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3], u[0]) + u[5] * u[5]) + u[6] * u[6]) +
              u[7] * u[7];
   pr = (p.gm - 1.0) * (u[4] - 0.5 * q);
   pr = dmax(pr, p.pr_lim);
-  u[4] = pr / (p.gm - 1.0) + 0.5 * q;
-  te = (0.5 * p.gm) * (pr / u[0]);
+  u[4] = fdiv(pr, p.gm - 1.0) + 0.5 * q;
+  te = (0.5 * p.gm) * fdiv(pr, u[0]);
 }
 
 // v2u_1: HLL_rot.f90:1189-1215
@@ -542,10 +588,10 @@ PIP_HD IRConst ir_constants(const Phys& p) {
 // Gm_rec, Gm_ion of one cell from rho_p and T_p: PIP_rot.f90:215-217
 PIP_HD void ir_rates(double rho_p, double te_p, const Phys& p, const IRConst& c, double& gr,
                      double& gi) {
-  gr = rho_p / sqrt(te_p) * p.t_ir / c.f_p * c.sqrt_tfac;
-  double a = 13.6 / c.Te_0 / te_p * c.tfac;
+  gr = fdiv(fdiv(rho_p, fsqrt(te_p)) * p.t_ir, c.f_p) * c.sqrt_tfac;
+  double a = fdiv(13.6 / c.Te_0, te_p) * c.tfac;
   double g = c.k_ion * rho_p * pexp(-a) * ppow(a, 0.39);
-  gi = g / (0.232 + a) / c.rec_fac / c.f_p * p.t_ir;
+  gi = fdiv(fdiv(fdiv(g, 0.232 + a), c.rec_fac), c.f_p) * p.t_ir;
 }
 
 // set_collisional, one cell, col_type 1..6 (PIP_rot.f90:44-97): get_Te_MHD
@@ -650,9 +696,9 @@ PIP_HD void source_pip(double* um, double* uh, double ac, double gr, double gi, 
     double kin = 0.5 * (gr * de * ((vx * vx + vy * vy) + vz * vz) -
                         gi * nde * ((nvx * nvx + nvy * nvy) + nvz * nvz));
     if (p.flag_ir_type == 2)
-      ds[4] = kin - gi * npr / (p.gm - 1.0) + gr * pr / (p.gm - 1.0);
+      ds[4] = kin - fdiv(gi * npr, p.gm - 1.0) + fdiv(gr * pr, p.gm - 1.0);
     else
-      ds[4] = kin - gi * npr / (p.gm - 1.0) + 0.5 * gr * pr / (p.gm - 1.0);
+      ds[4] = kin - fdiv(gi * npr, p.gm - 1.0) + fdiv(0.5 * gr * pr, p.gm - 1.0);
 #pragma unroll
     for (int n = 0; n < 5; n++) {
       Sh[n] = Sh[n] + ds[n];
@@ -677,13 +723,13 @@ PIP_HD double cfl_cell(double* um, double* uh, double gmin_safety, const Phys& p
   const double dd = dmax(de, kTiny), rdd = qrcp(dd);
   double cs2 = quo(p.gm * pr, dd, rdd);
   double valf2 = quo((bx * bx + by * by) + bz * bz, dd, rdd);
-  double vabs = sqrt((vx * vx + vy * vy) + vz * vz);
-  double dtm = qdiv(gmin_safety, vabs + sqrt(cs2 + valf2));
+  double vabs = fsqrt((vx * vx + vy * vy) + vz * vz);
+  double dtm = qdiv(gmin_safety, vabs + fsqrt(cs2 + valf2));
   cq2pv_hd(uh, ph, p);
-  double cs = sqrt(qdiv(p.gm * ph[4], ph[0]));
-  double vabsn = sqrt(dmax((ph[1] * ph[1] + ph[2] * ph[2]) + ph[3] * ph[3], kTiny));
+  double cs = fsqrt(qdiv(p.gm * ph[4], ph[0]));
+  double vabsn = fsqrt(dmax((ph[1] * ph[1] + ph[2] * ph[2]) + ph[3] * ph[3], kTiny));
   double dth = qdiv(gmin_safety, vabsn + cs);
   return dmin(dtm, dth);
 }
 
-}  // namespace pip
+}  // namespace PIP_NS

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -17,6 +18,7 @@
 #include "../../include/pipgpu.h"
 #include "pip_adjoint.h"
 #include "pip_kernels.h"
+#include "pip_relaxed.h"
 
 using namespace pip;
 
@@ -88,6 +90,7 @@ struct pip_ctx {
   int tx = 16, ty = 15;
   int xcd_remap = 0;  // contiguous tile ranges per XCD (autotuned)
   int two_pass = 0;   // bit s-1: stage s takes the fluids one at a time (autotuned)
+  int arith = PIP_ARITH_EXACT;  // stage kernels: exact (reference order) or relaxed build
   Phys ph;
   IRConst irc;
   Bnd bnd;
@@ -100,6 +103,12 @@ struct pip_ctx {
   double* adjV = nullptr;   // nvt planes
   double* adjP = nullptr;   // 2 dirs x nvt planes of (left, right) face-state pairs + 4 planes of routing codes
   double* lamB = nullptr;   // nvt planes
+  // pip_advance replays a captured chunk of fused steps (hipGraph); the key
+  // names the stage kernels it was captured with
+  int step_graph = 1;
+  hipGraphExec_t sgraph = nullptr;
+  long long sgraph_key = -1;
+  long long eager_key = -1;  // kernel choice of the last chunk run from the stream (attributes set)
 };
 
 // --------------------------------------------------------------- helpers
@@ -235,6 +244,15 @@ static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
 
 static int launch_stage(pip_ctx* c, const StageArgs& a) {
   const bool ir = c->cfg.flag_ir >= 2;
+  if (c->arith == PIP_ARITH_RELAXED && !a.dtc) {  // frozen-dt replays stay exact (adjoint)
+    const int tx = (c->cfg.ndim == 1) ? 255 : c->tx, ty = (c->cfg.ndim == 1) ? 1 : c->ty;
+    const char* err = "";
+    const int rc = pip_relaxed_stage(&a, sizeof(a), c->cfg.ndim, c->nv, ir ? 1 : 0, tx, ty, (void*)c->stream, &err);
+    if (rc == -1) return fail(PIP_ERR_UNSUPPORTED, "relaxed arithmetic: tiles 16x15, 32x7 and 16x7 (2-D)");
+    if (rc == -2) return fail(PIP_ERR_STATE, "relaxed arithmetic: stage argument block mismatch");
+    if (rc != 0) return fail(PIP_ERR_HIP, std::string("relaxed stage kernel: ") + err);
+    return PIP_OK;
+  }
   if (c->cfg.ndim == 1) {
     if (c->nv != 8) return fail(PIP_ERR_UNSUPPORTED, "1-D runs have nvar_m = 8");
     return ir ? launch_stage_ext<1, 8, true>(c, a) : launch_stage_ext<1, 8, false>(c, a);
@@ -450,6 +468,7 @@ int pip_destroy(pip_ctx* c) {
   (void)hipFree(c->adjV);
   (void)hipFree(c->adjP);
   (void)hipFree(c->lamB);
+  if (c->sgraph) (void)hipGraphExecDestroy(c->sgraph);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PIP_OK;
@@ -566,6 +585,44 @@ static int enqueue_fused(pip_ctx* c, int n) {
   return PIP_OK;
 }
 
+// The stage-kernel choice a captured chunk depends on.
+static long long step_graph_key(const pip_ctx* c) {
+  return ((((long long)c->tx * 64 + c->ty) * 2 + c->xcd_remap) * 8 + c->two_pass) * 2 + c->arith;
+}
+
+// One chunk of `n` fused steps from the captured graph (captured on first
+// use, after a chunk with the same kernel choice has run from the stream so
+// that no kernel attribute is set during capture, and again when the choice
+// changed); the kernels predicate on the device's active flag, so the chunk
+// is the same work as enqueue_fused(c, n).
+static int launch_step_graph(pip_ctx* c, int n) {
+  const long long key = step_graph_key(c);
+  if (!c->sgraph || c->sgraph_key != key) {
+    if (c->sgraph) {
+      HIPCHK(hipGraphExecDestroy(c->sgraph));
+      c->sgraph = nullptr;
+    }
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_fused(c, n);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (rc != PIP_OK) {
+      if (ce == hipSuccess) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ce != hipSuccess) return fail(PIP_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&c->sgraph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      c->sgraph = nullptr;
+      return fail(PIP_ERR_HIP, std::string("step graph instantiate: ") + hipGetErrorString(ie));
+    }
+    c->sgraph_key = key;
+  }
+  HIPCHK(hipGraphLaunch(c->sgraph, c->stream));
+  return PIP_OK;
+}
+
 int pip_advance(pip_ctx* c, int max_steps, double tend, int* steps_done, double* time) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
   if (max_steps < 0) return fail(PIP_ERR_ARG, "max_steps < 0");
@@ -579,7 +636,12 @@ int pip_advance(pip_ctx* c, int max_steps, double tend, int* steps_done, double*
   int left = max_steps;
   while (left > 0) {
     int n = left < chunk ? left : chunk;
-    TRY(enqueue_fused(c, n));
+    if (n == chunk && c->step_graph && c->eager_key == step_graph_key(c)) {
+      TRY(launch_step_graph(c, n));
+    } else {
+      TRY(enqueue_fused(c, n));
+      c->eager_key = step_graph_key(c);
+    }
     left -= n;
     HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -705,6 +767,27 @@ int pip_set_two_pass(pip_ctx* c, int stage_mask) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
   if (stage_mask < 0 || stage_mask > 7) return fail(PIP_ERR_ARG, "stage mask must be in 0..7");
   c->two_pass = stage_mask;
+  return PIP_OK;
+}
+
+int pip_set_arith(pip_ctx* c, int mode) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (mode != PIP_ARITH_EXACT && mode != PIP_ARITH_RELAXED) return fail(PIP_ERR_ARG, "arith mode must be 0 or 1");
+  if (mode == PIP_ARITH_RELAXED) {
+    const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
+    const bool ir = c->cfg.flag_ir >= 2;
+    if (ext || !pip_relaxed_covers(c->cfg.ndim, c->nv, ir ? 1 : 0, c->cfg.ndim == 1 ? 255 : 16, c->cfg.ndim == 1 ? 1 : 15))
+      return fail(PIP_ERR_UNSUPPORTED,
+                  "relaxed arithmetic covers 2-D runs with psi (with or without set_IR) and 1-D runs without "
+                  "set_IR, with constant or per-cell alpha_c and explicit collisions");
+  }
+  c->arith = mode;
+  return PIP_OK;
+}
+
+int pip_set_step_graph(pip_ctx* c, int on) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  c->step_graph = on ? 1 : 0;
   return PIP_OK;
 }
 
@@ -980,6 +1063,8 @@ int pip_adj_step_noalloc(pip_ctx* c, const double* Un, const double* U1, const d
 struct pip_ens {
   pip_cfg base;
   int n;
+  bool twod = false;  // 2-D members (psi): lockstep modes on 16x15 tiles
+  int nvt = 13;       // planes per member: nvar_m + 5
   std::vector<EnsMember> hmem;
   EnsMember* mem = nullptr;
   EnsOut* out = nullptr;
@@ -1003,6 +1088,12 @@ struct pip_ens {
   int* list = nullptr;              // lockstep: members of the current table
   int* h_list = nullptr;            // pinned staging of the list
   int max_tiles = 0;
+  // graph-captured lockstep (mode 2): one 64-step chunk over the fixed table
+  // of every member's tiles, replayed until no member runs
+  hipGraphExec_t lexec = nullptr;
+  int lexec_tx = 0;
+  int* d_active = nullptr;          // members still running after the chunk
+  int* h_active = nullptr;          // pinned copy (written by the graph)
 };
 
 template <int TX>
@@ -1039,28 +1130,26 @@ static int ens_launch(pip_ens* e) {
   return PIP_OK;
 }
 
-// Lockstep run: start kernel, then chunks of steps over the tile table of
-// the members still running, each step two stage launches (stage 1 applies
-// the previous step's end), each chunk closed by k_lock_end over its
-// members; the table is rebuilt between chunks from the members' active
-// flags.  A member once stopped is never listed again (its other parity
-// buffer may still hold its last running state).
-template <int TX>
-static int ens_lockstep_t(pip_ens* e) {
-  constexpr int TC = TX - 1;
-  using G = TileGeom<1, 8, TC, 1>;
-  const size_t lds_stage = sizeof(double) * (size_t)G::LDS_DOUBLES;
-  const size_t lds_start = sizeof(double) * (size_t)(G::LDS_DOUBLES + 3 * (TX / 64));
-  static bool attr = false;
-  if (!attr) {
-    HIPCHK(hipFuncSetAttribute((const void*)k_lock_start<TX>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_start));
-    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_stage));
-    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_stage));
-    attr = true;
+// Lockstep drivers.  Members' tiles are listed in a (member, tile) table:
+// 1-D members in tiles of tc cells, 2-D members in 16x15 tiles.
+static int ens_table(pip_ens* e, const std::vector<char>& run, int tc, int* nm_out) {
+  int nt = 0, nm = 0;
+  for (int m = 0; m < e->n; m++) {
+    if (!run[m]) continue;
+    e->h_list[nm++] = m;
+    const EnsMember& M = e->hmem[m];
+    const int ty = e->twod ? kEns2TY : 1, tx = e->twod ? kEns2TX : tc, ny = e->twod ? M.ny : 1;
+    for (int j0 = 0; j0 < ny; j0 += ty)
+      for (int i0 = 0; i0 < M.nx; i0 += tx) {
+        if (nt >= e->max_tiles) return -1;
+        e->h_tiles[nt++] = LockTile{m, i0, j0, 0};
+      }
   }
+  *nm_out = nm;
+  return nt;
+}
+
+static EnsArgs ens_args(pip_ens* e) {
   EnsArgs a;
   a.U = e->U;
   a.U1 = e->U1;
@@ -1072,30 +1161,39 @@ static int ens_lockstep_t(pip_ens* e) {
   a.ph = e->ph;
   a.bnd = e->bnd;
   a.db_clean = e->base.db_clean;
-  hipLaunchKernelGGL(k_lock_start<TX>, dim3((unsigned)e->n), dim3(TX), lds_start, e->stream, a, e->scs);
+  return a;
+}
+
+// Kernels of one kind of member: start(a), stage(istep, ntiles, p, apply_end).
+struct EnsKernels {
+  std::function<void(const EnsArgs&)> start;
+  std::function<void(const EnsArgs&, int, int, int, int)> stage;
+};
+
+// Host-driven lockstep: start kernel, then chunks of steps over the tile
+// table of the members still running, each step two stage launches (stage 1
+// applies the previous step's end), each chunk closed by k_lock_end over its
+// members; the table is rebuilt between chunks from the members' active
+// flags.  A member once stopped is never listed again (its other parity
+// buffer may still hold its last running state).
+static int ens_lockstep_run(pip_ens* e, int tc, const EnsKernels& K) {
+  const EnsArgs a = ens_args(e);
+  K.start(a);
   HIPCHK(hipGetLastError());
   std::vector<char> run(e->n, 1);  // members that may still run (all, until the start kernel has said)
   constexpr int chunk = 64;        // even: a chunk ends on the parity it started with
   long long k = 0;                 // global step index (parity of the scalar buffers)
   for (;;) {
-    int nt = 0, nm = 0;
-    for (int m = 0; m < e->n; m++) {
-      if (!run[m]) continue;
-      e->h_list[nm++] = m;
-      for (int i0 = 0; i0 < e->hmem[m].nx; i0 += TC) {
-        if (nt >= e->max_tiles) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
-        e->h_tiles[nt++] = LockTile{m, i0};
-      }
-    }
+    int nm = 0;
+    const int nt = ens_table(e, run, tc, &nm);
+    if (nt < 0) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
     if (nt == 0) break;
     HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)nm, hipMemcpyHostToDevice, e->stream));
     for (int s = 0; s < chunk; s++) {
       const int p = (int)((k + s) & 1);
-      hipLaunchKernelGGL((k_lock_stage<TX, 1>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
-                         e->tiles, p, s > 0 ? 1 : 0);
-      hipLaunchKernelGGL((k_lock_stage<TX, 2>), dim3((unsigned)nt), dim3(TX), lds_stage, e->stream, a, e->scs,
-                         e->tiles, p, 0);
+      K.stage(a, 1, nt, p, s > 0 ? 1 : 0);
+      K.stage(a, 2, nt, p, 0);
     }
     const int pl = (int)((k + chunk - 1) & 1);
     hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, e->stream, a, e->scs, e->list,
@@ -1113,15 +1211,148 @@ static int ens_lockstep_t(pip_ens* e) {
   return PIP_OK;
 }
 
-static int ens_lockstep(pip_ens* e) {
-  switch (e->tx) {
-    case 64: return ens_lockstep_t<64>(e);
-    case 128: return ens_lockstep_t<128>(e);
-    case 256: return ens_lockstep_t<256>(e);
-    case 512: return ens_lockstep_t<512>(e);
+// Graph-captured lockstep: the start kernel, then replays of one captured
+// chunk -- kLockChunk steps of the two stage launches over the fixed table
+// of every member's tiles (members that have stopped return at once: their
+// scalars say inactive), the chunk's closing step end over all members, the
+// count of members still running and its copy to pinned host memory.  The
+// host replays kGraphBatch chunks between looks at the count; no table is
+// rebuilt.  Same kernels and order as lockstep mode, so the same results.
+constexpr int kLockChunk = 64;  // even: a chunk ends on the parity it started with
+constexpr int kGraphBatch = 4;
+
+static int ens_lockgraph_run(pip_ens* e, int tc, int key, const EnsKernels& K) {
+  const EnsArgs a = ens_args(e);
+  if (!e->lexec || e->lexec_tx != key) {
+    if (e->lexec) {
+      HIPCHK(hipGraphExecDestroy(e->lexec));
+      e->lexec = nullptr;
+    }
+    std::vector<char> all(e->n, 1);
+    int nm = 0;
+    const int nt = ens_table(e, all, tc, &nm);
+    if (nt < 0) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
+    HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)e->n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < kLockChunk; s++) {
+      const int p = s & 1;
+      K.stage(a, 1, nt, p, s > 0 ? 1 : 0);
+      K.stage(a, 2, nt, p, 0);
+    }
+    hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((e->n + 255) / 256)), dim3(256), 0, e->stream, a, e->scs,
+                       e->list, e->n, (kLockChunk - 1) & 1);
+    hipLaunchKernelGGL(k_lock_count, dim3(1), dim3(256), 0, e->stream, e->scs, e->n, e->d_active);
+    const hipError_t le = hipGetLastError();
+    const hipError_t me = hipMemcpyAsync(e->h_active, e->d_active, sizeof(int), hipMemcpyDeviceToHost, e->stream);
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (le != hipSuccess || me != hipSuccess || ce != hipSuccess) {
+      if (ce == hipSuccess) (void)hipGraphDestroy(g);
+      const hipError_t bad = le != hipSuccess ? le : (me != hipSuccess ? me : ce);
+      return fail(PIP_ERR_HIP, std::string("lockstep graph capture: ") + hipGetErrorString(bad));
+    }
+    const hipError_t ie = hipGraphInstantiate(&e->lexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      e->lexec = nullptr;
+      return fail(PIP_ERR_HIP, std::string("lockstep graph instantiate: ") + hipGetErrorString(ie));
+    }
+    e->lexec_tx = key;
   }
-  return fail(PIP_ERR_ARG, "ensemble tile must be 64, 128, 256 or 512");
+  K.start(a);
+  HIPCHK(hipGetLastError());
+  for (;;) {
+    for (int b = 0; b < kGraphBatch; b++) HIPCHK(hipGraphLaunch(e->lexec, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (*e->h_active == 0) break;
+  }
+  return PIP_OK;
 }
+
+// The kernels of 1-D members at width TX (tiles of TX - 1 cells).
+template <int TX>
+static int ens_kernels_1d(pip_ens* e, EnsKernels* K) {
+  using G = TileGeom<1, 8, TX - 1, 1>;
+  const size_t lds_stage = sizeof(double) * (size_t)G::LDS_DOUBLES;
+  const size_t lds_start = sizeof(double) * (size_t)(G::LDS_DOUBLES + 3 * (TX / 64));
+  static bool attr = false;
+  if (!attr) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_start<TX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_start));
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage<TX, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    attr = true;
+  }
+  hipStream_t st = e->stream;
+  DevScalars* scs = e->scs;
+  const LockTile* tiles = e->tiles;
+  const int n = e->n;
+  K->start = [=](const EnsArgs& a) {
+    hipLaunchKernelGGL(k_lock_start<TX>, dim3((unsigned)n), dim3(TX), lds_start, st, a, scs);
+  };
+  K->stage = [=](const EnsArgs& a, int istep, int nt, int p, int apply_end) {
+    if (istep == 1)
+      hipLaunchKernelGGL((k_lock_stage<TX, 1>), dim3((unsigned)nt), dim3(TX), lds_stage, st, a, scs, tiles, p, apply_end);
+    else
+      hipLaunchKernelGGL((k_lock_stage<TX, 2>), dim3((unsigned)nt), dim3(TX), lds_stage, st, a, scs, tiles, p, apply_end);
+  };
+  return PIP_OK;
+}
+
+// The kernels of 2-D members (16x15 tiles).
+static int ens_kernels_2d(pip_ens* e, EnsKernels* K) {
+  const size_t lds_stage = sizeof(double) * (size_t)TileGeom<2, 9, kEns2TX, kEns2TY>::LDS_DOUBLES;
+  static bool attr = false;
+  if (!attr) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage2<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    HIPCHK(hipFuncSetAttribute((const void*)k_lock_stage2<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_stage));
+    attr = true;
+  }
+  hipStream_t st = e->stream;
+  DevScalars* scs = e->scs;
+  const LockTile* tiles = e->tiles;
+  const int n = e->n;
+  K->start = [=](const EnsArgs& a) {
+    hipLaunchKernelGGL(k_lock_start2<256>, dim3((unsigned)n), dim3(256), 0, st, a, scs);
+  };
+  K->stage = [=](const EnsArgs& a, int istep, int nt, int p, int apply_end) {
+    if (istep == 1)
+      hipLaunchKernelGGL(k_lock_stage2<1>, dim3((unsigned)nt), dim3(kEns2NT), lds_stage, st, a, scs, tiles, p, apply_end);
+    else
+      hipLaunchKernelGGL(k_lock_stage2<2>, dim3((unsigned)nt), dim3(kEns2NT), lds_stage, st, a, scs, tiles, p, apply_end);
+  };
+  return PIP_OK;
+}
+
+// Lockstep (graph = false) or graph-captured lockstep (graph = true) run of
+// all members.
+static int ens_lock_any(pip_ens* e, bool graph) {
+  EnsKernels K;
+  int tc = 0;
+  if (e->twod) {
+    TRY(ens_kernels_2d(e, &K));
+  } else {
+    switch (e->tx) {
+      case 64: TRY(ens_kernels_1d<64>(e, &K)); break;
+      case 128: TRY(ens_kernels_1d<128>(e, &K)); break;
+      case 256: TRY(ens_kernels_1d<256>(e, &K)); break;
+      case 512: TRY(ens_kernels_1d<512>(e, &K)); break;
+      default: return fail(PIP_ERR_ARG, "ensemble tile must be 64, 128, 256 or 512");
+    }
+    tc = e->tx - 1;
+  }
+  const int key = e->twod ? 1 : e->tx;  // what the captured graph was built for
+  return graph ? ens_lockgraph_run(e, tc, key, K) : ens_lockstep_run(e, tc, K);
+}
+
+static int ens_lockgraph(pip_ens* e) { return ens_lock_any(e, true); }
+static int ens_lockstep(pip_ens* e) { return ens_lock_any(e, false); }
 
 static int ens_enqueue(pip_ens* e) {
   switch (e->tx) {
@@ -1137,9 +1368,15 @@ static int ens_enqueue(pip_ens* e) {
 // graph, when asked for, is captured on first use and kept); *ms = the
 // device time between them.
 static int ens_run_between(pip_ens* e, int use_graph, hipEvent_t a, hipEvent_t b, float* ms) {
+  if (e->twod && e->mode == 0)
+    return fail(PIP_ERR_UNSUPPORTED, "2-D ensembles run in lockstep (mode 1) or graph-captured lockstep (mode 2)");
   if (e->mode == 1) {  // host-driven chunks: no graph
     HIPCHK(hipEventRecord(a, e->stream));
     TRY(ens_lockstep(e));
+    HIPCHK(hipEventRecord(b, e->stream));
+  } else if (e->mode == 2) {  // graph-captured lockstep chunks
+    HIPCHK(hipEventRecord(a, e->stream));
+    TRY(ens_lockgraph(e));
     HIPCHK(hipEventRecord(b, e->stream));
   } else if (use_graph) {
     if (!e->gexec || e->graph_tx != e->tx) {
@@ -1182,21 +1419,27 @@ extern "C" {
 int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pip_ens** out) {
   if (!base || !members || !out || n <= 0) return fail(PIP_ERR_ARG, "null argument or n <= 0");
   *out = nullptr;
-  if (base->ndim != 1 || base->nvar_m != 8) return fail(PIP_ERR_UNSUPPORTED, "ensemble members are 1-D (nvar_m = 8)");
+  const bool twod = base->ndim == 2;
+  if (!((base->ndim == 1 && base->nvar_m == 8) || (twod && base->nvar_m == 9 && base->flag_divb == 1)))
+    return fail(PIP_ERR_UNSUPPORTED, "ensemble members are 1-D (nvar_m = 8) or 2-D with psi (nvar_m = 9, flag_divb = 1)");
   if (base->flag_ir != 0 || base->ac_field || base->flag_col != 1 || base->flag_pip_imp != 0)
     return fail(PIP_ERR_UNSUPPORTED, "ensemble: flag_IR = 0, constant alpha_c, explicit collisions");
   if (base->t_order != 2) return fail(PIP_ERR_UNSUPPORTED, "ensemble: t_order = 2");
-  for (int s = 0; s < 2; s++) {
+  for (int s = 0; s < 2 * base->ndim; s++) {
     int t = bnd_kind(base->flag_bnd[s]);
     if (t != 1 && t != 2 && t != 10) return fail(PIP_ERR_UNSUPPORTED, "boundary flag must be 1, 2-7, 10 or 11");
   }
   for (int m = 0; m < n; m++) {
     if (members[m].nx < 4 || !(members[m].dx > 0)) return fail(PIP_ERR_ARG, "member nx >= 4 and dx > 0 required");
     if (members[m].nx + 4 > kMaxPad) return fail(PIP_ERR_ARG, "member nx must be <= 32763");
+    if (twod && (members[m].ny < 4 || !(members[m].dy > 0) || members[m].ny + 4 > kMaxPad))
+      return fail(PIP_ERR_ARG, "2-D members need 4 <= ny <= 32763 and dy > 0");
   }
   pip_ens* e = new pip_ens();
   e->base = *base;
   e->n = n;
+  e->twod = twod;
+  e->nvt = base->nvar_m + 5;
   e->ph.gm = base->gm;
   e->ph.ro_lim = base->ro_lim;
   e->ph.pr_lim = base->pr_lim;
@@ -1208,10 +1451,10 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   e->ph.flag_col = base->flag_col;
   e->ph.flag_ir = 0;
   e->ph.flag_ir_type = 0;
-  {  // boundary masks of a 1-D, nvar_m = 8 context
+  {  // boundary masks of a member context
     pip_ctx tmp;
     tmp.cfg = *base;
-    tmp.nv = 8;
+    tmp.nv = base->nvar_m;
     build_bnd(&tmp);
     e->bnd = tmp.bnd;
   }
@@ -1224,10 +1467,14 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
     M.ix = members[m].nx + 4;
     M.dx = members[m].dx;
     M.col = members[m].col;
-    M.gmin_safety = base->safety * dmin(dmin(M.dx, 100.0), 100.0);
+    M.ny = twod ? members[m].ny : 0;
+    M.jx = twod ? members[m].ny + 4 : 1;
+    M.dy = twod ? members[m].dy : 100.0;  // Model_rot.f90:424-429
+    M.dxyz_min = dmin(dmin(M.dx, M.dy), 100.0);
+    M.gmin_safety = base->safety * M.dxyz_min;
     M.tend = members[m].tend;
     M.nmax = members[m].max_steps > 0 ? members[m].max_steps : ((long long)1 << 62);
-    off += 13LL * M.ix;
+    off += (long long)e->nvt * M.ix * M.jx;
   }
   e->total = off;
   std::vector<int> order(n);
@@ -1251,7 +1498,9 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
   if ((er = hipMalloc(&e->out, sizeof(EnsOut) * n)) != hipSuccess) return bail("hipMalloc out");
   if ((er = hipMalloc(&e->order, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc order");
   if ((er = hipMalloc(&e->counter, sizeof(unsigned) * 4)) != hipSuccess) return bail("hipMalloc counter");
-  for (int m = 0; m < n; m++) e->max_tiles += (members[m].nx + 62) / 63;  // tiles of the narrowest width
+  for (int m = 0; m < n; m++)  // tiles of the narrowest 1-D width, or 16x15 tiles
+    e->max_tiles += twod ? ((members[m].nx + kEns2TX - 1) / kEns2TX) * ((members[m].ny + kEns2TY - 1) / kEns2TY)
+                         : (members[m].nx + 62) / 63;
   if ((er = hipMalloc(&e->scs, 2 * sizeof(DevScalars) * n)) != hipSuccess) return bail("hipMalloc scalars");
   if ((er = hipMalloc(&e->list, sizeof(int) * n)) != hipSuccess) return bail("hipMalloc list");
   if ((er = hipHostMalloc((void**)&e->h_list, sizeof(int) * n, hipHostMallocDefault)) != hipSuccess)
@@ -1261,6 +1510,9 @@ int pip_ens_create(const pip_cfg* base, const pip_ens_member* members, int n, pi
     return bail("hipHostMalloc tiles");
   if ((er = hipHostMalloc((void**)&e->h_scs, sizeof(DevScalars) * n, hipHostMallocDefault)) != hipSuccess)
     return bail("hipHostMalloc scalars");
+  if ((er = hipMalloc(&e->d_active, sizeof(int))) != hipSuccess) return bail("hipMalloc active");
+  if ((er = hipHostMalloc((void**)&e->h_active, sizeof(int), hipHostMallocDefault)) != hipSuccess)
+    return bail("hipHostMalloc active");
   if ((er = hipMemcpy(e->mem, e->hmem.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
   if ((er = hipMemcpy(e->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess) return bail("copy");
   if ((er = hipMemset(e->U, 0, sizeof(double) * off)) != hipSuccess) return bail("memset");
@@ -1275,6 +1527,9 @@ int pip_ens_destroy(pip_ens* e) {
   if (!e) return PIP_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->lexec) (void)hipGraphExecDestroy(e->lexec);
+  (void)hipFree(e->d_active);
+  if (e->h_active) (void)hipHostFree(e->h_active);
   (void)hipFree(e->U);
   (void)hipFree(e->U1);
   (void)hipFree(e->Uinit);
@@ -1296,20 +1551,24 @@ int pip_ens_destroy(pip_ens* e) {
 int pip_ens_upload(pip_ens* e, int m, const double* U_h, const double* U_m) {
   if (!e || !U_h || !U_m || m < 0 || m >= e->n) return fail(PIP_ERR_ARG, "bad member or null array");
   const EnsMember& M = e->hmem[m];
-  size_t pb = sizeof(double) * M.ix;
-  HIPCHK(hipMemcpy(e->Uinit + M.off, U_m, pb * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(e->Uinit + M.off + 8LL * M.ix, U_h, pb * 5, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(e->U + M.off, e->Uinit + M.off, pb * 13, hipMemcpyDeviceToDevice));
+  const long long N = (long long)M.ix * M.jx;
+  const int nv = e->nvt - 5;
+  const size_t pb = sizeof(double) * (size_t)N;
+  HIPCHK(hipMemcpy(e->Uinit + M.off, U_m, pb * nv, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->Uinit + M.off + nv * N, U_h, pb * 5, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->U + M.off, e->Uinit + M.off, pb * e->nvt, hipMemcpyDeviceToDevice));
   return PIP_OK;
 }
 
 int pip_ens_download(pip_ens* e, int m, double* U_h, double* U_m) {
   if (!e || !U_h || !U_m || m < 0 || m >= e->n) return fail(PIP_ERR_ARG, "bad member or null array");
   const EnsMember& M = e->hmem[m];
-  size_t pb = sizeof(double) * M.ix;
+  const long long N = (long long)M.ix * M.jx;
+  const int nv = e->nvt - 5;
+  const size_t pb = sizeof(double) * (size_t)N;
   HIPCHK(hipStreamSynchronize(e->stream));
-  HIPCHK(hipMemcpy(U_m, e->U + M.off, pb * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(U_h, e->U + M.off + 8LL * M.ix, pb * 5, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(U_m, e->U + M.off, pb * nv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(U_h, e->U + M.off + nv * N, pb * 5, hipMemcpyDeviceToHost));
   return PIP_OK;
 }
 
@@ -1322,7 +1581,8 @@ int pip_ens_set_tile(pip_ens* e, int tx) {
 
 int pip_ens_set_mode(pip_ens* e, int mode) {
   if (!e) return fail(PIP_ERR_ARG, "null ensemble");
-  if (mode != 0 && mode != 1) return fail(PIP_ERR_ARG, "ensemble mode must be 0 (persistent) or 1 (lockstep)");
+  if (mode < 0 || mode > 2)
+    return fail(PIP_ERR_ARG, "ensemble mode must be 0 (persistent), 1 (lockstep) or 2 (graph-captured lockstep)");
   e->mode = mode;
   return PIP_OK;
 }

@@ -1,5 +1,6 @@
-"""Batched ensembles of independent 1-D two-fluid runs in one persistent
-launch (the pip_ens_* entry points of include/pipgpu.h).
+"""Batched ensembles of independent 1-D two-fluid runs (the pip_ens_* entry
+points of include/pipgpu.h): one persistent launch, host-driven lockstep
+launches, or lockstep chunks captured as a hipGraph.
 
 Each member is a full run of the reference time loop (main.f90:31-68) with
 its own size, coefficients, dt sequence and end time.  Members are scheduled
@@ -15,6 +16,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
+
+
+MODES = {"persistent": 0, "lockstep": 1, "graph": 2}  # pip_ens_set_mode
 
 
 def _p(a):
@@ -38,10 +42,13 @@ def cost_estimate(cfg, U_h, U_m, tend: float) -> float:
 
 class Ensemble:
     def __init__(self, members, tend: float, max_steps: int = 0, tile: int = 256):
-        """members: list of (cfg, U_h, U_m) with 1-D configurations that share
-        gm, safety, limits and boundary flags."""
+        """members: list of (cfg, U_h, U_m) with configurations of one
+        dimension that share gm, safety, limits and boundary flags: 1-D, or
+        2-D with psi (flag_divb = 1; lockstep and graph modes)."""
         self.cfgs = [m[0] for m in members]
         base = self.cfgs[0]
+        if any(c.ndim != base.ndim or c.nvar_m != base.nvar_m for c in self.cfgs):
+            raise ValueError("ensemble members must share ndim and nvar_m")
         self._base = _lib.cfg_to_c(base)
         arr = (_lib.PipEnsMember * len(members))()
         for k, (cfg, uh, um) in enumerate(members):
@@ -50,7 +57,9 @@ class Ensemble:
             arr[k].col = cfg.col
             arr[k].tend = tend
             arr[k].max_steps = max_steps
-            arr[k].cost = float(cost_estimate(cfg, uh, um, tend))
+            arr[k].cost = float(cost_estimate(cfg, uh, um, tend)) if cfg.ndim == 1 else float(cfg.nx * cfg.ny)
+            arr[k].ny = cfg.ny if cfg.ndim == 2 else 0
+            arr[k].dy = cfg.dy if cfg.ndim == 2 else 0.0
         self._arr = arr
         h = ctypes.c_void_p()
         check(lib().pip_ens_create(ctypes.byref(self._base), arr, len(members), ctypes.byref(h)))
@@ -59,6 +68,8 @@ class Ensemble:
             check(lib().pip_ens_upload(self.h, k, _p(np.ascontiguousarray(uh)), _p(np.ascontiguousarray(um))))
         self.set_tile(tile)
         self.mode = "persistent"
+        if base.ndim == 2:
+            self.set_mode("lockstep")
 
     def close(self):
         if getattr(self, "h", None):
@@ -78,8 +89,11 @@ class Ensemble:
     def set_mode(self, mode: str):
         """'persistent': one launch, a workgroup per member (hipGraph replay
         with graph=True); 'lockstep': every member's tiles spread over
-        workgroups, two stage launches and a step end per step."""
-        check(lib().pip_ens_set_mode(self.h, {"persistent": 0, "lockstep": 1}[mode]))
+        workgroups, two stage launches and a step end per step, host-driven
+        64-step chunks; 'graph': the lockstep chunk captured once as a
+        hipGraph over every member's tiles and replayed until all members
+        stop."""
+        check(lib().pip_ens_set_mode(self.h, MODES[mode]))
         self.mode = mode
 
     def reset(self):
@@ -97,7 +111,7 @@ class Ensemble:
 
     def state(self, m: int):
         cfg = self.cfgs[m]
-        uh = np.empty((5, 1, cfg.ix))
-        um = np.empty((8, 1, cfg.ix))
+        uh = np.empty((5, cfg.jx, cfg.ix))
+        um = np.empty((cfg.nvar_m, cfg.jx, cfg.ix))
         check(lib().pip_ens_download(self.h, m, _p(uh), _p(um)))
         return uh, um

@@ -23,12 +23,18 @@ from . import _lib
 from ._lib import check, lib
 
 
+ARITH = {"exact": 0, "relaxed": 1}  # PIP_ARITH_EXACT, PIP_ARITH_RELAXED
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 class PIPSolver:
-    def __init__(self, cfg, U_h, U_m, ac=None, prologue=True, tile=None):
+    def __init__(self, cfg, U_h, U_m, ac=None, prologue=True, tile=None, arith="exact"):
+        """arith: "exact" (the reference's arithmetic, bitwise the CPU path)
+        or "relaxed" (FMA contraction, reciprocal-multiply quotients,
+        hardware min/max; see pip_set_arith)."""
         self.cfg = cfg
         self._ccfg = _lib.cfg_to_c(cfg)
         h = ctypes.c_void_p()
@@ -37,6 +43,7 @@ class PIPSolver:
         self.time_host = 0.0
         if tile is not None:
             self.set_tile(*tile)  # (tx, ty[, xcd_remap[, two_pass stage mask]])
+        self.set_arith(arith)
         self.upload(U_h, U_m, ac)
         if prologue:
             self.prologue()
@@ -167,6 +174,10 @@ class PIPSolver:
             check(lib().pip_set_xcd_remap(self.h, int(xcd_remap)))
         if two_pass is not None:
             check(lib().pip_set_two_pass(self.h, int(two_pass)))
+
+    def set_arith(self, arith: str):
+        check(lib().pip_set_arith(self.h, ARITH[arith]))
+        self.arith = arith
 
     def time_steps(self, nsteps: int):
         tt, t1, t2 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -1,6 +1,7 @@
-"""Batched ensemble (one persistent launch, optional hipGraph replay, or
-lockstep launches with every member spread over workgroups) equals the plain
-per-member runs and the oracle bit for bit."""
+"""Batched ensemble (one persistent launch, optional hipGraph replay;
+lockstep launches with every member spread over workgroups; or lockstep
+chunks captured as a hipGraph) equals the plain per-member runs and the
+oracle bit for bit."""
 import numpy as np
 import pytest
 
@@ -31,7 +32,8 @@ MAXS = 20000
 
 @pytest.mark.parametrize("mode,graph,tile", [("persistent", True, 256), ("persistent", False, 64),
                                              ("persistent", True, 512), ("lockstep", False, 256),
-                                             ("lockstep", False, 64), ("lockstep", False, 512)])
+                                             ("lockstep", False, 64), ("lockstep", False, 512),
+                                             ("graph", False, 256), ("graph", False, 64), ("graph", False, 512)])
 def test_ensemble_matches_plain_runs(mode, graph, tile):
     print("case ensemble 10 members mode", mode, "graph", graph, "tile", tile, flush=True)
     mem = _members(10)
@@ -78,15 +80,16 @@ def test_lockstep_equals_persistent_at_the_step_cap(cap):
     ens.run(graph=False)
     ref = [(ens.result(k), ens.state(k)) for k in range(12)]
     assert all(r["nt"] == cap for r, _ in ref)
-    ens.set_mode("lockstep")
-    for _ in range(2):
-        ens.reset()
-        ens.run()
-        for k in range(12):
-            r, (uh, um) = ref[k]
-            assert ens.result(k) == r, k
-            eh, em = ens.state(k)
-            assert np.array_equal(eh, uh) and np.array_equal(em, um), k
+    for mode in ("lockstep", "graph"):
+        ens.set_mode(mode)
+        for _ in range(2):
+            ens.reset()
+            ens.run()
+            for k in range(12):
+                r, (uh, um) = ref[k]
+                assert ens.result(k) == r, (mode, k)
+                eh, em = ens.state(k)
+                assert np.array_equal(eh, uh) and np.array_equal(em, um), (mode, k)
 
 
 def test_lockstep_and_persistent_agree_on_48_members():
@@ -99,15 +102,16 @@ def test_lockstep_and_persistent_agree_on_48_members():
     ens.run(graph=True)
     ref = [(ens.result(k), ens.state(k)) for k in range(48)]
     assert len({r["nt"] for r, _ in ref}) > 10  # members stop at many different steps
-    ens.set_mode("lockstep")
-    ens.set_tile(128)
-    ens.reset()
-    ens.run()
-    for k in range(48):
-        r, (uh, um) = ref[k]
-        assert ens.result(k) == r, k
-        eh, em = ens.state(k)
-        assert np.array_equal(eh, uh) and np.array_equal(em, um), k
+    for mode in ("lockstep", "graph"):
+        ens.set_mode(mode)
+        ens.set_tile(128)
+        ens.reset()
+        ens.run()
+        for k in range(48):
+            r, (uh, um) = ref[k]
+            assert ens.result(k) == r, (mode, k)
+            eh, em = ens.state(k)
+            assert np.array_equal(eh, uh) and np.array_equal(em, um), (mode, k)
 
 
 @pytest.mark.timeout(600)
@@ -131,7 +135,7 @@ def test_ensemble_members_match_the_oracle():
         ref.append((o.scalars(), o.state()))
         print(f"  oracle member {check[k]}: nx {cfg.nx}, {ref[-1][0]['nt']} steps", flush=True)
     ens = Ensemble(mem, tend=0.1, max_steps=MAXS, tile=256)
-    for mode in ("persistent", "lockstep"):
+    for mode in ("persistent", "lockstep", "graph"):
         ens.set_mode(mode)
         ens.reset()
         ens.run(graph=True)
@@ -143,4 +147,56 @@ def test_ensemble_members_match_the_oracle():
             assert np.array_equal(em[:, :, 2:-2], om[:, :, 2:-2]), (mode, check[k])
             assert np.array_equal(eh[:, :, 2:-2], oh[:, :, 2:-2]), (mode, check[k])
         print(f"  {mode}: bitwise equal", flush=True)
+    ens.close()
+
+
+def _members_2d():
+    """Uneven 2-D two-fluid members (Orszag-Tang initial states, psi) of
+    sizes that are not multiples of the 16x15 tile, sweeping the neutral
+    fraction and the collision coefficient."""
+    out = []
+    for n, xi, col in ((24, 0.9, 1.0e3), (40, 0.5, 30.0), (33, 0.2, 3.0e3), (64, 0.7, 100.0),
+                       (17, 0.95, 10.0), (50, 0.3, 1.0e4)):
+        cfg, uh, um, _ = problems.orszag_tang(n, xi_n=xi, col=col)
+        out.append((cfg, uh, um))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["lockstep", "graph"])
+def test_2d_ensemble_matches_plain_and_oracle(mode):
+    """2-D members (psi, constant alpha_c) of six sizes run 90 steps in one
+    ensemble: every member equals its plain pip_advance run bit for bit
+    (ghosts excluded), time and step count included; two members also
+    equal the oracle's run."""
+    print("case 2-D ensemble, 6 members, 90 steps, mode", mode, flush=True)
+    mem = _members_2d()
+    ens = Ensemble(mem, tend=1.0e30, max_steps=90)
+    ens.set_mode(mode)
+    ens.run()
+    for k, (cfg, uh, um) in enumerate(mem):
+        s = PIPSolver(cfg, uh, um)
+        n, t = s.advance(nsteps=90)
+        r = ens.result(k)
+        assert r["status"] == 0 and r["nt"] == n == 90 and r["time"] == t, (k, r, n, t)
+        eh, em = ens.state(k)
+        ph, pm = s.state()
+        assert np.array_equal(em[:, 2:-2, 2:-2], pm[:, 2:-2, 2:-2]), k
+        assert np.array_equal(eh[:, 2:-2, 2:-2], ph[:, 2:-2, 2:-2]), k
+        s.close()
+        if k in (1, 2):
+            o = Oracle(cfg, uh, um)
+            o.run(nsteps=90)
+            oh, om = o.state()
+            assert np.array_equal(em[:, 2:-2, 2:-2], om[:, 2:-2, 2:-2]), k
+            assert np.array_equal(eh[:, 2:-2, 2:-2], oh[:, 2:-2, 2:-2]), k
+    ens.close()
+
+
+def test_2d_ensemble_rejects_the_persistent_mode():
+    print("case 2-D ensemble persistent mode rejected", flush=True)
+    from pip_amd._lib import PipError
+    ens = Ensemble(_members_2d()[:2], tend=1.0, max_steps=5)
+    ens.set_mode("persistent")
+    with pytest.raises(PipError, match="2-D ensembles run in lockstep"):
+        ens.run()
     ens.close()
