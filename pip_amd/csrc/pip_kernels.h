@@ -269,23 +269,38 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
     if (i0 + fi > a.nx || j0 + fj >= a.ny) continue;
     int hy = fj + OY;
     const double* row = V + hy * HX;
-    double vl[NVT], vr[NVT];
+    // one fluid at a time: the neutral face states are not live while the
+    // plasma solver runs (same operations, fewer registers at its peak)
+    {
+      double vl[NV], vr[NV], fl[NV];
 #pragma unroll
-    for (int n = 0; n < NVT; n++) {
-      const double* r = row + n * HY * HX + fi;
-      double a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
-      vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
-      vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+      for (int n = 0; n < NV; n++) {
+        const double* r = row + n * HY * HX + fi;
+        double a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+        vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+        vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+      }
+      vl[0] = dmax(vl[0], a.ph.ro_lim);
+      vr[0] = dmax(vr[0], a.ph.ro_lim);
+      hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+#pragma unroll
+      for (int n = 0; n < NV; n++) F[n * NF + f] = fl[n];
     }
-    vl[0] = dmax(vl[0], a.ph.ro_lim);
-    vr[0] = dmax(vr[0], a.ph.ro_lim);
-    vl[NV] = dmax(vl[NV], a.ph.ro_lim);
-    vr[NV] = dmax(vr[NV], a.ph.ro_lim);
-    double fl[NVT];
-    hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
-    hllc_flux(vl + NV, vr + NV, a.ph.gm, fl + NV);
+    {
+      double vl[5], vr[5], fl[5];
 #pragma unroll
-    for (int n = 0; n < NVT; n++) F[n * NF + f] = fl[n];
+      for (int n = 0; n < 5; n++) {
+        const double* r = row + (NV + n) * HY * HX + fi;
+        double a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+        vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+        vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+      }
+      vl[0] = dmax(vl[0], a.ph.ro_lim);
+      vr[0] = dmax(vr[0], a.ph.ro_lim);
+      hllc_flux(vl, vr, a.ph.gm, fl);
+#pragma unroll
+      for (int n = 0; n < 5; n++) F[(NV + n) * NF + f] = fl[n];
+    }
   }
   __syncthreads();
 
@@ -351,34 +366,51 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
       int fi = f % TX, fj = f / TX;
       if (i0 + fi >= a.nx || j0 + fj > a.ny) continue;
       int hx = fi + 2;
-      double vl[NVT], vr[NVT];
+      // permutation (1,3,2,4,5,7,6,8,9)/(1,3,2,4,5): HLL_rot.f90:414-434,
+      // 315-325; one fluid at a time as in the x faces
+      {
+        double vl[NV], vr[NV], fl[NV];
 #pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        // permutation (1,3,2,4,5,7,6,8,9)/(1,3,2,4,5): HLL_rot.f90:414-434, 315-325
-        int src = n;
-        if (n == 1) src = 2; else if (n == 2) src = 1;
-        else if (n == 5) src = 6; else if (n == 6) src = 5;
-        else if (n == NV + 1) src = NV + 2; else if (n == NV + 2) src = NV + 1;
-        const double* col = V + src * HY * HX + hx;
-        double a0 = col[(fj + 0) * HX], a1 = col[(fj + 1) * HX], a2 = col[(fj + 2) * HX],
-               a3 = col[(fj + 3) * HX];
-        vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
-        vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+        for (int n = 0; n < NV; n++) {
+          int src = n;
+          if (n == 1) src = 2; else if (n == 2) src = 1;
+          else if (n == 5) src = 6; else if (n == 6) src = 5;
+          const double* col = V + src * HY * HX + hx;
+          double a0 = col[(fj + 0) * HX], a1 = col[(fj + 1) * HX], a2 = col[(fj + 2) * HX],
+                 a3 = col[(fj + 3) * HX];
+          vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+          vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+        }
+        vl[0] = dmax(vl[0], a.ph.ro_lim);
+        vr[0] = dmax(vr[0], a.ph.ro_lim);
+        hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          int dst = n;
+          if (n == 1) dst = 2; else if (n == 2) dst = 1;
+          else if (n == 5) dst = 6; else if (n == 6) dst = 5;
+          F[dst * NF + f] = fl[n];
+        }
       }
-      vl[0] = dmax(vl[0], a.ph.ro_lim);
-      vr[0] = dmax(vr[0], a.ph.ro_lim);
-      vl[NV] = dmax(vl[NV], a.ph.ro_lim);
-      vr[NV] = dmax(vr[NV], a.ph.ro_lim);
-      double fl[NVT];
-      hlld_flux<NV>(vl, vr, a.ph.gm, cmax, fl);
-      hllc_flux(vl + NV, vr + NV, a.ph.gm, fl + NV);
+      {
+        double vl[5], vr[5], fl[5];
 #pragma unroll
-      for (int n = 0; n < NVT; n++) {
-        int dst = n;
-        if (n == 1) dst = 2; else if (n == 2) dst = 1;
-        else if (n == 5) dst = 6; else if (n == 6) dst = 5;
-        else if (n == NV + 1) dst = NV + 2; else if (n == NV + 2) dst = NV + 1;
-        F[dst * NF + f] = fl[n];
+        for (int n = 0; n < 5; n++) {
+          int src = (n == 1) ? 2 : (n == 2) ? 1 : n;
+          const double* col = V + (NV + src) * HY * HX + hx;
+          double a0 = col[(fj + 0) * HX], a1 = col[(fj + 1) * HX], a2 = col[(fj + 2) * HX],
+                 a3 = col[(fj + 3) * HX];
+          vl[n] = a1 + 0.5 * minmod(a0, a1, a2);
+          vr[n] = a2 - 0.5 * minmod(a1, a2, a3);
+        }
+        vl[0] = dmax(vl[0], a.ph.ro_lim);
+        vr[0] = dmax(vr[0], a.ph.ro_lim);
+        hllc_flux(vl, vr, a.ph.gm, fl);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          int dst = (n == 1) ? 2 : (n == 2) ? 1 : n;
+          F[(NV + dst) * NF + f] = fl[n];
+        }
       }
     }
     __syncthreads();
