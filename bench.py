@@ -398,12 +398,15 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     s = PIPSolver(cfg, uh, um, ac, tile=tile)
     s.advance(5)  # warm-up
     s.close()
-    s = PIPSolver(cfg, uh, um, ac, tile=tile)
-    s.sync()
-    t0 = time.perf_counter()
-    s.advance(nsteps)
-    t_fwd = time.perf_counter() - t0
-    s.close()
+    fwd = []
+    for _ in range(3):
+        s = PIPSolver(cfg, uh, um, ac, tile=tile)
+        s.sync()
+        t0 = time.perf_counter()
+        s.advance(nsteps)
+        fwd.append(time.perf_counter() - t0)
+        s.close()
+    t_fwd = float(np.median(fwd))
     dev = torch.device("cuda:0")
     run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile)
     U0 = torch.tensor(np.concatenate([um, uh]), device=dev)
@@ -411,7 +414,7 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     mask = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
     mask[2:-2, 2:-2] = 1.0
     times = []
-    for _ in range(2):  # the first pass allocates the adjoint buffers
+    for _ in range(4):  # the first pass allocates the adjoint buffers
         U = U0.clone().requires_grad_(True)
         a = acT.clone().requires_grad_(True)
         torch.cuda.synchronize()
@@ -423,10 +426,11 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         times.append((t1 - t0, t2 - t1))
-    t_rec, t_bwd = times[-1]
+    t_rec = float(np.median([t[0] for t in times[1:]]))
+    t_bwd = float(np.median([t[1] for t in times[1:]]))
     run.close()
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
-            "tile": list(tile),
+            "tile": list(tile), "timing": "medians of 3 runs each",
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
             "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd}
 

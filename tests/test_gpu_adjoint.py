@@ -313,9 +313,10 @@ def _directional(cfg, run, base, v0, ac, Ustar, mask, g_ac, g_v, d, eps):
 # jumps of the derivative there.  Directions smooth on the grid scale barely
 # move those margins.  An eps sweep (1e-8 .. 1e-3, four seeds) with
 # Gaussian-filtered noise of width ~25 cells gives FD/adjoint agreement
-# within 1e-7 over the whole plateau eps = 1e-7 .. 3e-5 for both parameters;
-# with width ~1.3 cells the error grows in proportion to eps from ~1e-7
-# (branch flips) and, below eps ~ 1e-7, as 1/eps (round-off).
+# within 1e-7 for eps = 1e-7 .. 1e-6 and within 4e-7 up to 1e-4 for both
+# parameters; with width ~1.3 cells the error grows in proportion to eps
+# (branch flips, up to 5e-3 at eps = 1e-4) and, below eps ~ 1e-7, as 1/eps
+# (round-off).
 SMOOTH_FILT = 160.0   # problems.smooth_field filter: width 160 / (2 pi) ~ 25 cells
 BUMP_SIGMA = 25.0     # cells
 

@@ -117,7 +117,7 @@ def test_lockstep_and_persistent_agree_on_48_members():
 @pytest.mark.timeout(600)
 def test_ensemble_members_match_the_oracle():
     """Members of the 256-member ensemble (seed 7) run to t = 0.1 inside a
-    24-member ensemble, both launch modes, against the oracle's runs of the
+    24-member ensemble, in all three launch modes, against the oracle's runs of the
     same members: the stiffest (alpha_c = 9.8e3), the largest (nx = 4089),
     the smallest (nx = 260) and two others -- every interior cell, the step
     count and the end time bitwise."""
