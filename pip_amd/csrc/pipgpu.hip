@@ -1088,13 +1088,30 @@ struct pip_ens {
   int* list = nullptr;              // lockstep: members of the current table
   int* h_list = nullptr;            // pinned staging of the list
   int max_tiles = 0;
-  // graph-captured lockstep (mode 2): one 64-step chunk over the fixed table
-  // of every member's tiles, replayed until no member runs
-  hipGraphExec_t lexec = nullptr;
-  int lexec_tx = 0;
+  // graph-captured lockstep (mode 2): a 64-step chunk over the tile table of
+  // a set of members, captured once per set and replayed; the set shrinks to
+  // the members still running when half of it has stopped
+  struct LockGraph {
+    std::vector<char> run;          // the members the table lists
+    hipGraphExec_t exec = nullptr;
+    LockTile* tiles = nullptr;      // this graph's own table and member list
+    int* list = nullptr;
+    int nm = 0;
+  };
+  std::vector<LockGraph> lgraphs;
+  int lgraph_key = 0;               // kernel choice the cached graphs were built for
   int* d_active = nullptr;          // members still running after the chunk
   int* h_active = nullptr;          // pinned copy (written by the graph)
 };
+
+static void ens_drop_graphs(pip_ens* e) {
+  for (auto& g : e->lgraphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    (void)hipFree(g.tiles);
+    (void)hipFree(g.list);
+  }
+  e->lgraphs.clear();
+}
 
 template <int TX>
 static int ens_launch(pip_ens* e) {
@@ -1164,10 +1181,11 @@ static EnsArgs ens_args(pip_ens* e) {
   return a;
 }
 
-// Kernels of one kind of member: start(a), stage(istep, ntiles, p, apply_end).
+// Kernels of one kind of member: start(a), stage(a, table, istep, ntiles, p,
+// apply_end).
 struct EnsKernels {
   std::function<void(const EnsArgs&)> start;
-  std::function<void(const EnsArgs&, int, int, int, int)> stage;
+  std::function<void(const EnsArgs&, const LockTile*, int, int, int, int)> stage;
 };
 
 // Host-driven lockstep: start kernel, then chunks of steps over the tile
@@ -1192,8 +1210,8 @@ static int ens_lockstep_run(pip_ens* e, int tc, const EnsKernels& K) {
     HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)nm, hipMemcpyHostToDevice, e->stream));
     for (int s = 0; s < chunk; s++) {
       const int p = (int)((k + s) & 1);
-      K.stage(a, 1, nt, p, s > 0 ? 1 : 0);
-      K.stage(a, 2, nt, p, 0);
+      K.stage(a, e->tiles, 1, nt, p, s > 0 ? 1 : 0);
+      K.stage(a, e->tiles, 2, nt, p, 0);
     }
     const int pl = (int)((k + chunk - 1) & 1);
     hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, e->stream, a, e->scs, e->list,
@@ -1211,62 +1229,102 @@ static int ens_lockstep_run(pip_ens* e, int tc, const EnsKernels& K) {
   return PIP_OK;
 }
 
-// Graph-captured lockstep: the start kernel, then replays of one captured
-// chunk -- kLockChunk steps of the two stage launches over the fixed table
-// of every member's tiles (members that have stopped return at once: their
-// scalars say inactive), the chunk's closing step end over all members, the
-// count of members still running and its copy to pinned host memory.  The
-// host replays kGraphBatch chunks between looks at the count; no table is
-// rebuilt.  Same kernels and order as lockstep mode, so the same results.
+// Graph-captured lockstep: the start kernel, then replays of a captured
+// chunk -- kLockChunk steps of the two stage launches over the tile table of
+// a set of members (members of the set that have stopped return at once:
+// their scalars say inactive), the chunk's closing step end over the set,
+// the count of all members still running and its copy to pinned host
+// memory.  The host replays kGraphBatch chunks between looks at the count.
+// When half of the set has stopped, the set shrinks to the members still
+// running: their graph is taken from the cache or captured (with its own
+// table), so a rerun of the same ensemble captures nothing.  Same kernels and
+// order as lockstep mode, so the same results.
 constexpr int kLockChunk = 64;  // even: a chunk ends on the parity it started with
 constexpr int kGraphBatch = 4;
+constexpr int kGraphCacheMax = 64;
+
+static int ens_lockgraph_get(pip_ens* e, const EnsArgs& a, int tc, const EnsKernels& K,
+                             const std::vector<char>& run, int* idx) {
+  for (size_t g = 0; g < e->lgraphs.size(); g++)
+    if (e->lgraphs[g].run == run) {
+      *idx = (int)g;
+      return PIP_OK;
+    }
+  if ((int)e->lgraphs.size() >= kGraphCacheMax) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    ens_drop_graphs(e);
+  }
+  int nm = 0;
+  const int nt = ens_table(e, run, tc, &nm);
+  if (nt < 0) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
+  pip_ens::LockGraph G;
+  G.run = run;
+  G.nm = nm;
+  if (nt > 0) {
+    HIPCHK(hipMalloc(&G.tiles, sizeof(LockTile) * (size_t)nt));
+    HIPCHK(hipMalloc(&G.list, sizeof(int) * (size_t)nm));
+    HIPCHK(hipMemcpyAsync(G.tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(G.list, e->h_list, sizeof(int) * (size_t)nm, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  if (nt > 0) {
+    for (int s = 0; s < kLockChunk; s++) {
+      const int p = s & 1;
+      K.stage(a, G.tiles, 1, nt, p, s > 0 ? 1 : 0);
+      K.stage(a, G.tiles, 2, nt, p, 0);
+    }
+    hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, e->stream, a, e->scs, G.list,
+                       nm, (kLockChunk - 1) & 1);
+  }
+  hipLaunchKernelGGL(k_lock_count, dim3(1), dim3(256), 0, e->stream, e->scs, e->n, e->d_active);
+  const hipError_t le = hipGetLastError();
+  const hipError_t me = hipMemcpyAsync(e->h_active, e->d_active, sizeof(int), hipMemcpyDeviceToHost, e->stream);
+  const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+  hipError_t bad = le != hipSuccess ? le : (me != hipSuccess ? me : ce);
+  if (bad == hipSuccess) {
+    bad = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+    if (bad != hipSuccess) G.exec = nullptr;
+  }
+  if (ce == hipSuccess) (void)hipGraphDestroy(g);
+  if (bad != hipSuccess) {
+    (void)hipFree(G.tiles);
+    (void)hipFree(G.list);
+    return fail(PIP_ERR_HIP, std::string("lockstep graph capture: ") + hipGetErrorString(bad));
+  }
+  e->lgraphs.push_back(std::move(G));
+  *idx = (int)e->lgraphs.size() - 1;
+  return PIP_OK;
+}
 
 static int ens_lockgraph_run(pip_ens* e, int tc, int key, const EnsKernels& K) {
   const EnsArgs a = ens_args(e);
-  if (!e->lexec || e->lexec_tx != key) {
-    if (e->lexec) {
-      HIPCHK(hipGraphExecDestroy(e->lexec));
-      e->lexec = nullptr;
-    }
-    std::vector<char> all(e->n, 1);
-    int nm = 0;
-    const int nt = ens_table(e, all, tc, &nm);
-    if (nt < 0) return fail(PIP_ERR_STATE, "lockstep tile table overflow");
-    HIPCHK(hipMemcpyAsync(e->tiles, e->h_tiles, sizeof(LockTile) * (size_t)nt, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->list, e->h_list, sizeof(int) * (size_t)e->n, hipMemcpyHostToDevice, e->stream));
+  if (e->lgraph_key != key) {
     HIPCHK(hipStreamSynchronize(e->stream));
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < kLockChunk; s++) {
-      const int p = s & 1;
-      K.stage(a, 1, nt, p, s > 0 ? 1 : 0);
-      K.stage(a, 2, nt, p, 0);
-    }
-    hipLaunchKernelGGL(k_lock_end, dim3((unsigned)((e->n + 255) / 256)), dim3(256), 0, e->stream, a, e->scs,
-                       e->list, e->n, (kLockChunk - 1) & 1);
-    hipLaunchKernelGGL(k_lock_count, dim3(1), dim3(256), 0, e->stream, e->scs, e->n, e->d_active);
-    const hipError_t le = hipGetLastError();
-    const hipError_t me = hipMemcpyAsync(e->h_active, e->d_active, sizeof(int), hipMemcpyDeviceToHost, e->stream);
-    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-    if (le != hipSuccess || me != hipSuccess || ce != hipSuccess) {
-      if (ce == hipSuccess) (void)hipGraphDestroy(g);
-      const hipError_t bad = le != hipSuccess ? le : (me != hipSuccess ? me : ce);
-      return fail(PIP_ERR_HIP, std::string("lockstep graph capture: ") + hipGetErrorString(bad));
-    }
-    const hipError_t ie = hipGraphInstantiate(&e->lexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-      e->lexec = nullptr;
-      return fail(PIP_ERR_HIP, std::string("lockstep graph instantiate: ") + hipGetErrorString(ie));
-    }
-    e->lexec_tx = key;
+    ens_drop_graphs(e);
+    e->lgraph_key = key;
   }
+  std::vector<char> run(e->n, 1);  // the set: every member, until half of them have stopped
+  int cur = 0;
+  TRY(ens_lockgraph_get(e, a, tc, K, run, &cur));
+  int listed = e->lgraphs[cur].nm;
   K.start(a);
   HIPCHK(hipGetLastError());
   for (;;) {
-    for (int b = 0; b < kGraphBatch; b++) HIPCHK(hipGraphLaunch(e->lexec, e->stream));
+    for (int b = 0; b < kGraphBatch; b++) HIPCHK(hipGraphLaunch(e->lgraphs[cur].exec, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (*e->h_active == 0) break;
+    const int active = *e->h_active;
+    if (active == 0) break;
+    if (2 * active <= listed) {
+      // the chunk's end wrote every listed member's scalars to buffer 0
+      HIPCHK(hipMemcpyAsync(e->h_scs, e->scs, sizeof(DevScalars) * (size_t)e->n, hipMemcpyDeviceToHost,
+                            e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      for (int m = 0; m < e->n; m++) run[m] = (char)(run[m] && e->h_scs[m].active != 0);
+      TRY(ens_lockgraph_get(e, a, tc, K, run, &cur));
+      listed = e->lgraphs[cur].nm;
+    }
   }
   return PIP_OK;
 }
@@ -1289,12 +1347,11 @@ static int ens_kernels_1d(pip_ens* e, EnsKernels* K) {
   }
   hipStream_t st = e->stream;
   DevScalars* scs = e->scs;
-  const LockTile* tiles = e->tiles;
   const int n = e->n;
   K->start = [=](const EnsArgs& a) {
     hipLaunchKernelGGL(k_lock_start<TX>, dim3((unsigned)n), dim3(TX), lds_start, st, a, scs);
   };
-  K->stage = [=](const EnsArgs& a, int istep, int nt, int p, int apply_end) {
+  K->stage = [=](const EnsArgs& a, const LockTile* tiles, int istep, int nt, int p, int apply_end) {
     if (istep == 1)
       hipLaunchKernelGGL((k_lock_stage<TX, 1>), dim3((unsigned)nt), dim3(TX), lds_stage, st, a, scs, tiles, p, apply_end);
     else
@@ -1316,12 +1373,11 @@ static int ens_kernels_2d(pip_ens* e, EnsKernels* K) {
   }
   hipStream_t st = e->stream;
   DevScalars* scs = e->scs;
-  const LockTile* tiles = e->tiles;
   const int n = e->n;
   K->start = [=](const EnsArgs& a) {
     hipLaunchKernelGGL(k_lock_start2<256>, dim3((unsigned)n), dim3(256), 0, st, a, scs);
   };
-  K->stage = [=](const EnsArgs& a, int istep, int nt, int p, int apply_end) {
+  K->stage = [=](const EnsArgs& a, const LockTile* tiles, int istep, int nt, int p, int apply_end) {
     if (istep == 1)
       hipLaunchKernelGGL(k_lock_stage2<1>, dim3((unsigned)nt), dim3(kEns2NT), lds_stage, st, a, scs, tiles, p, apply_end);
     else
@@ -1527,7 +1583,7 @@ int pip_ens_destroy(pip_ens* e) {
   if (!e) return PIP_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  if (e->lexec) (void)hipGraphExecDestroy(e->lexec);
+  ens_drop_graphs(e);
   (void)hipFree(e->d_active);
   if (e->h_active) (void)hipHostFree(e->h_active);
   (void)hipFree(e->U);

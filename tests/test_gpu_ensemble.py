@@ -54,17 +54,23 @@ def test_ensemble_matches_plain_runs(mode, graph, tile):
         s.close()
 
 
-def test_ensemble_rerun_after_reset_is_identical():
-    print("case ensemble reset/rerun", flush=True)
+@pytest.mark.parametrize("mode", ["persistent", "lockstep", "graph"])
+def test_ensemble_rerun_after_reset_is_identical(mode):
+    """A rerun from the initial states repeats the first run bit for bit; in
+    graph mode the rerun replays the cached graphs of the shrinking member
+    sets the first run captured."""
+    print("case ensemble reset/rerun", mode, flush=True)
     mem = _members(6, nx_cap=600)
     ens = Ensemble(mem, tend=0.05, max_steps=MAXS)
+    ens.set_mode(mode)
     ens.run(graph=True)
-    a = [ens.state(k) for k in range(6)]
+    a = [(ens.result(k), ens.state(k)) for k in range(6)]
     ens.reset()
     ens.run(graph=True)
     for k in range(6):
-        b = ens.state(k)
-        assert np.array_equal(a[k][0], b[0]) and np.array_equal(a[k][1], b[1])
+        r, b = ens.result(k), ens.state(k)
+        assert r == a[k][0]
+        assert np.array_equal(a[k][1][0], b[0]) and np.array_equal(a[k][1][1], b[1])
 
 
 @pytest.mark.parametrize("cap", [150, 128, 64])
