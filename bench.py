@@ -182,7 +182,7 @@ def write_sq_summary(d: str, wl: str, n: int, tile: str, out: str, arith: str = 
     with the FP64 VALU ceiling they imply."""
     main_args = f"--workload {wl} --n {n} --steps 10 --warmup 3 --no-cpu --no-extra --tile {tile} --arith {arith}"
     cmds = [f"cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc {c} --output-format csv "
-            f"-d /root/repo/gpurun_out/pmc/{sub} -o run -- python3 /root/repo/bench.py {main_args}"
+            f"-d {d}/{sub} -o run -- python3 /root/repo/bench.py {main_args}"
             for sub, c in SQ_PASSES]
     vals = {}
     for sub, cs in SQ_PASSES:
