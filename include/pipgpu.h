@@ -226,10 +226,12 @@ int pip_ens_set_tile(pip_ens *e, int tx);
  * workgroups, two stage launches per step for all running members (stage 1
  * applies the previous step's end), host-driven in 64-step chunks with the
  * tile table rebuilt between chunks (use_graph ignored); 2: graph-captured
- * lockstep, one 64-step chunk over the fixed table of every member's tiles
- * captured once as a hipGraph (stopped members' tiles return at once) and
- * replayed until no member runs, no table rebuild (use_graph ignored).  All
- * give every member's plain pip_advance run bit for bit. */
+ * lockstep, a 64-step chunk over the tile table of a set of members captured
+ * as a hipGraph and replayed until no member runs (stopped members' tiles
+ * return at once); when half of the set has stopped the set shrinks to the
+ * members still running, each set's graph captured once and cached
+ * (use_graph ignored).  All give every member's plain pip_advance run bit
+ * for bit. */
 int pip_ens_set_mode(pip_ens *e, int mode);
 /* restore every member to its uploaded state (device copy, not timed) */
 int pip_ens_reset(pip_ens *e);
