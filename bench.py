@@ -100,6 +100,21 @@ def traffic_from_profile(wl: str, n: int, arith: str = "exact"):
             "stage2": num("stage2_bytes_per_launch"), "tile": t.group(1) if t else None}
 
 
+def valu_from_profile(wl: str, n: int, arith: str = "exact"):
+    """(VALU lane-instructions per cell-update, the FP64 VALU ceiling in
+    cell-updates/s they imply) from the committed SQ counter summary."""
+    tag = "" if arith == "exact" else f"_{arith}"
+    path = os.path.join(ROOT, "profiles", f"k_stage_{WL_NAME[wl]}_{n}{tag}_sq.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    li = re.search(r"lane-instructions per cell-update \(both stages\):\s*([0-9.eE+]+)", txt)
+    ce = re.search(r"FP64 VALU ceiling .*?:\s*([0-9.eE+]+) cell-updates/s", txt)
+    if not (li and ce):
+        return None
+    return float(li.group(1)), float(ce.group(1))
+
+
 def _csv_rows(d: str, pattern: str):
     import csv
     import glob
@@ -547,6 +562,11 @@ def main():
         "bytes_per_cell_update": r["b_cell"],
         "device_ms_per_step": r["device_ms_per_step"],
     }
+    valu = valu_from_profile(args.workload, n, arith)
+    if valu:
+        # the bound in practice: FP64 VALU issue (profiles/k_stage_*_sq.md)
+        out["valu_ceiling"] = {"lane_instr_per_cell_update": valu[0], "cell_updates_per_s": valu[1],
+                               "frac": r["value"] / valu[1]}
     for a in ariths:
         if a == arith:
             continue
@@ -558,6 +578,10 @@ def main():
                               "stage1_ms": r1["t1_ms"], "stage2_ms": r1["t2_ms"],
                               "traffic": tr["mean"] if tr else None,
                               "hbm_roofline_cell_updates_frac": r1["value"] * r1["b_cell"] / (HBM_PEAK_GBS * 1e9)}
+        v1 = valu_from_profile(args.workload, n, a)
+        if v1:
+            out["other_arith"]["valu_ceiling"] = {"lane_instr_per_cell_update": v1[0], "cell_updates_per_s": v1[1],
+                                                  "frac": r1["value"] / v1[1]}
     if not args.no_extra:
         if args.workload == "kh":
             c2, t2, r2 = plain_leg("ot", 1024, 500, 20, "auto", arith)

@@ -3,7 +3,8 @@ sizes and step counts, bitwise:
 
   * 2-D two-fluid Orszag-Tang, 1024^2 periodic, 500 steps (the whole run);
   * 2-D two-fluid Kelvin-Helmholtz with set_IR case 3 (ion_pot / arb_heat),
-    2048^2, x periodic, y mirror, the first 100 of its 1000 steps;
+    2048^2, x periodic, y mirror, the first 100 of its 1000 steps, and the
+    same configuration at 384^2 for all 1000 steps;
   * 2-D Alfven-wave damping with a per-cell alpha_c field, 512^2, 200 steps
     (the adjoint configuration's forward run).
 
@@ -96,6 +97,15 @@ def test_kelvin_helmholtz_ir_2048_100_steps():
     say("case kelvin_helmholtz n=2048 steps=100 vs oracle (IR case 3, mirror y)")
     cfg, uh, um, ac = problems.kelvin_helmholtz(2048)
     _bitwise_run(cfg, uh, um, ac, 100)
+
+
+@pytest.mark.timeout(600)
+def test_kelvin_helmholtz_ir_384_1000_steps():
+    """The Kelvin-Helmholtz + IR configuration for its whole 1000 steps, at
+    384^2 so that the oracle finishes in about a minute."""
+    say("case kelvin_helmholtz n=384 steps=1000 vs oracle (IR case 3, mirror y)")
+    cfg, uh, um, ac = problems.kelvin_helmholtz(384)
+    _bitwise_run(cfg, uh, um, ac, 1000)
 
 
 @pytest.mark.timeout(600)
