@@ -31,13 +31,25 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# The stage kernels are scheduled to group their memory instructions into
+# clauses (measured: ~1 % faster on the 2048^2 Kelvin-Helmholtz + IR stage
+# than the default schedule, same registers, same results).  The adjoint's
+# gather would take 346 VGPRs under it (one wave per SIMD instead of three),
+# so the adjoint kernels' unit keeps the default schedule.
+STAGE_SCHED = ["-mllvm", "--amdgpu-sched-strategy=max-memory-clause"]
 # Per translation unit:
-#   pipgpu.hip          no FMA contraction: every product and sum is rounded
-#                       separately, as in the reference built for x86-64; the
-#                       kernels reproduce the reference arithmetic bit for bit
-#   pipgpu_relaxed.hip  the relaxed build of the stage kernel: contraction on
-HIP_SOURCES = [("pipgpu.hip", ["-ffp-contract=off"]),
-               ("pipgpu_relaxed.hip", ["-ffp-contract=fast"])]
+#   pipgpu.hip          the C-ABI and the forward, ensemble and bookkeeping
+#                       kernels; no FMA contraction: every product and sum is
+#                       rounded separately, as in the reference built for
+#                       x86-64, so the kernels reproduce the reference
+#                       arithmetic bit for bit
+#   pipgpu_adjoint.hip  the adjoint kernels (no contraction outside the VJP
+#                       bodies, which enable it themselves)
+#   pipgpu_relaxed.hip  the relaxed build of the stage kernel: contraction
+#                       on, reciprocal-multiply quotients (not bitwise)
+HIP_SOURCES = [("pipgpu.hip", ["-ffp-contract=off", *STAGE_SCHED]),
+               ("pipgpu_adjoint.hip", ["-ffp-contract=off"]),
+               ("pipgpu_relaxed.hip", ["-ffp-contract=fast", *STAGE_SCHED])]
 
 
 def _run(cmd, cwd=None):

@@ -1,10 +1,10 @@
 // pip_adjoint.h -- HIP kernels of the discrete adjoint stage (one work item
-// of pip_adj_items.h per thread) and the step bookkeeping of recorded and
-// replayed runs.
+// of pip_adj_items.h per thread).  Compiled in pipgpu_adjoint.hip only.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "pip_adj_items.h"
-#include "pip_kernels.h"
 
 namespace pip {
 
@@ -96,33 +96,5 @@ __global__ void __launch_bounds__(256) k_zero_ghosts(double* x, int nvt, int ix,
   zero_ghosts_item(x, nvt, ix, jx, N, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-
-// Step bookkeeping for recorded / replayed runs: hist[2k] = dt, hist[2k+1]
-// = cmax of step k.
-__global__ void k_record_dt(const DevScalars* sc, double* hist, int k) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  hist[2 * k] = sc->active ? sc->dt : 0.0;
-  hist[2 * k + 1] = sc->cmax;
-}
-// End of a frozen-dt replay of n steps: the scalars a step-by-step run
-// leaves (time and step count advanced, dt / cmax of the last step).
-__global__ void k_replay_end(DevScalars* sc, const double* hist, int n) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double t = sc->time;
-  for (int k = 0; k < n; k++) t = t + hist[2 * k];
-  sc->time = t;
-  sc->nt = sc->nt + n;
-  sc->dt = hist[2 * (n - 1)];
-  sc->cmax = hist[2 * (n - 1) + 1];
-  sc->active = 1;
-  sc->last = 0;
-}
-__global__ void k_set_dt(DevScalars* sc, const double* hist, int k) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  sc->dt = hist[2 * k];
-  sc->cmax = hist[2 * k + 1];
-  sc->active = 1;
-  sc->last = 0;
-}
 
 }  // namespace pip

@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "../../include/pipgpu.h"
-#include "pip_adjoint.h"
+#include "pip_adj_launch.h"
 #include "pip_kernels.h"
 #include "pip_relaxed.h"
 
@@ -898,13 +898,7 @@ static AdjArgs adj_args(pip_ctx* c, int istep, const double* Uin, const double* 
 static int adj_stage(pip_ctx* c, const AdjArgs& a) {
   // the primitives, when the replay did not save them, and (then) the stage-1
   // ghost zeroing of lam_in; otherwise the gather's extra blocks zero them
-  if (a.make_V) hipLaunchKernelGGL(k_adj_cell, cell_grid(c), dim3(256), 0, c->stream, a);
-  const dim3 gf((unsigned)(adj_face_blocks(c->nx + 1, c->ny) + adj_face_blocks(c->nx, c->ny + 1)));
-  hipLaunchKernelGGL(k_adj_faces<0>, gf, dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_adj_faces<1>, gf, dim3(256), 0, c->stream, a);
-  const int ghost = (a.istep == 1 && !a.make_V) ? adj_ghost_blocks(c->cfg.ix, c->cfg.jx) : 0;
-  hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)(adj_tiles(c->nx, c->ny) + ghost)), dim3(256), 0, c->stream, a);
-  HIPCHK(hipGetLastError());
+  HIPCHK(adj_stage_launch(a, c->stream));
   return PIP_OK;
 }
 
@@ -997,10 +991,7 @@ int pip_replay_steps(pip_ctx* c, int n, const double* hist, double* Un, double* 
 
 int pip_adj_seed(pip_ctx* c, double* lam) {
   if (!c || !lam) return fail(PIP_ERR_ARG, "null argument");
-  hipLaunchKernelGGL(k_adj_fold_seed, cell_grid(c), dim3(256), 0, c->stream, lam, c->nvt, c->cfg.ix, c->cfg.jx,
-                     c->N, c->bnd);
-  hipLaunchKernelGGL(k_zero_ghosts, cell_grid(c), dim3(256), 0, c->stream, lam, c->nvt, c->cfg.ix, c->cfg.jx, c->N);
-  HIPCHK(hipGetLastError());
+  HIPCHK(adj_seed_launch(lam, c->nvt, c->cfg.ix, c->cfg.jx, c->N, c->bnd, c->stream));
   return PIP_OK;
 }
 
