@@ -11,6 +11,7 @@ module pipgpu
 
   integer(c_int), parameter :: PIP_OK = 0
   integer(c_int), parameter :: PIP_STOP_DT_SMALL = 1, PIP_STOP_NAN = 2
+  integer(c_int), parameter :: PIP_ARITH_EXACT = 0, PIP_ARITH_RELAXED = 1
 
   ! pip_cfg of include/pipgpu.h
   type, bind(c) :: pip_cfg
@@ -82,6 +83,24 @@ module pipgpu
     end function
     type(c_ptr) function pip_last_error() bind(c, name="pip_last_error")
       import :: c_ptr
+    end function
+    ! Tuning and variants (no reference counterpart): the stage tile, the
+    ! arithmetic (PIP_ARITH_EXACT, bitwise the reference path, or
+    ! PIP_ARITH_RELAXED) and the hipGraph replay of pip_advance's chunks.
+    integer(c_int) function pip_set_tile(ctx, tx, ty) bind(c, name="pip_set_tile")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tx, ty
+    end function
+    integer(c_int) function pip_set_arith(ctx, mode) bind(c, name="pip_set_arith")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function pip_set_step_graph(ctx, on) bind(c, name="pip_set_step_graph")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
     end function
   end interface
 
