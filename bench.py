@@ -380,6 +380,52 @@ def batched_leg(n_members: int = 256, tend: float = 0.1, max_steps: int = 20000)
             "unit": "cell-updates/s", "speedup": wall_seq / wall_ens, "bitwise_identical": bool(identical)}
 
 
+def batched_2d_leg(n_members: int = 64, steps: int = 200):
+    """Ensemble of n_members uneven 2-D two-fluid runs (Orszag-Tang states
+    with psi, n x n with n log-uniform in [32, 256], xi_n and alpha_c swept;
+    seed 9), each for `steps` steps: the graph-captured lockstep ensemble vs
+    sequential plain pip_advance runs of the same members."""
+    import torch
+    from pip_amd import problems
+    from pip_amd.ensemble import Ensemble
+    from pip_amd.solver import PIPSolver
+    mem = [problems.ensemble_member_2d(p)[:3] for p in problems.ensemble_members_2d(n_members)]
+    ens = Ensemble(mem, tend=1.0e30, max_steps=steps)
+    timing = {}
+    for mode in ("lockstep", "graph"):
+        ens.set_mode(mode)
+        ens.reset()
+        ens.run()  # graph mode: captures and caches its graphs
+        ens.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.run()
+        timing[mode] = time.perf_counter() - t0
+    updates, wall_seq, identical = 0, 0.0, True
+    for k, (cfg, uh, um) in enumerate(mem):
+        s = PIPSolver(cfg, uh, um)
+        s.sync()
+        t0 = time.perf_counter()
+        n, t = s.advance(nsteps=steps)
+        wall_seq += time.perf_counter() - t0
+        updates += cfg.nx * cfg.ny * n
+        r = ens.result(k)
+        eh, em = ens.state(k)
+        ph, pm = s.state()
+        i = (slice(None), slice(2, -2), slice(2, -2))
+        identical &= (r["nt"] == n and r["time"] == t and np.array_equal(em[i], pm[i])
+                      and np.array_equal(eh[i], ph[i]))
+        s.close()
+    ens.close()
+    sizes = [c.nx for c, _, _ in mem]
+    return {"workload": f"{n_members} x 2-D two-fluid Orszag-Tang (psi), n x n with n log-uniform in "
+                        f"[32,256] ({min(sizes)}..{max(sizes)}), xi_n and alpha_c swept, {steps} steps each",
+            "cell_updates": updates, "mode": "graph", "ensemble_s": timing["graph"],
+            "lockstep_s": timing["lockstep"], "sequential_plain_s": wall_seq,
+            "ensemble_value": updates / timing["graph"], "sequential_value": updates / wall_seq,
+            "unit": "cell-updates/s", "speedup": wall_seq / timing["graph"], "bitwise_identical": bool(identical)}
+
+
 def ensemble_leg(mode: str, tile: int, n_members: int = 256, tend: float = 0.1, max_steps: int = 20000):
     """The batched leg's ensemble alone in one launch mode and width, run
     twice from its initial state (for profiles)."""
@@ -475,7 +521,7 @@ def main():
                     help="write the SQ counter summary of the passes under DIR (sq1, sq2) to OUT and exit")
     ap.add_argument("--stats-cmd", default="", help="command recorded in --stats-summary")
     ap.add_argument("--stats-note", default="", help="note recorded in --stats-summary")
-    ap.add_argument("--only", choices=["batched", "adjoint", "copy", "ensemble"],
+    ap.add_argument("--only", choices=["batched", "batched_2d", "adjoint", "copy", "ensemble"],
                     help="run one extra leg alone (profiling); copy = 1 GiB device copy that "
                          "calibrates the HBM byte counters; ensemble = the batched leg's ensemble "
                          "alone, twice, in --ens-mode at --ens-tile (no autotuning, no plain runs)")
@@ -522,7 +568,8 @@ def main():
         print(json.dumps({"ensemble": ensemble_leg(args.ens_mode, args.ens_tile)}))
         return
     if args.only:
-        print(json.dumps({args.only: batched_leg() if args.only == "batched" else adjoint_leg()}))
+        legs = {"batched": batched_leg, "batched_2d": batched_2d_leg, "adjoint": adjoint_leg}
+        print(json.dumps({args.only: legs[args.only]()}))
         return
 
     ariths = ["exact", "relaxed"] if args.arith == "both" else [args.arith]
@@ -592,6 +639,7 @@ def main():
                                 "roofline_frac": r2["ach_GBs"] / HBM_PEAK_GBS,
                                 "hbm_roofline_cell_updates_frac": r2["value"] * r2["b_cell"] / (HBM_PEAK_GBS * 1e9)}
         out["batched"] = batched_leg()
+        out["batched_2d"] = batched_2d_leg()
         out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, n, args.cpu_steps, 1)

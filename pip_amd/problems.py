@@ -233,3 +233,19 @@ def ensemble_members(n_members: int = 256, seed: int = 7):
 def ensemble_member(params) -> Arrays:
     return brio_wu(nx=params["nx"], xi_n=params["xi_n"], col=params["col"],
                    p_left=params["p_left"], p_right=params["p_right"])
+
+
+def ensemble_members_2d(n_members: int = 64, seed: int = 9):
+    """Member parameters of a batched 2-D ensemble (Orszag-Tang states with
+    psi): n x n cells with n = floor(32*8^u) in [32, 256], xi_n ~ U[0.1,0.99],
+    ac = 10^(4u) -- a sweep over neutral fraction and collision coefficient
+    on uneven grids."""
+    u = uniform01(seed, 3 * n_members).reshape(3, n_members)
+    n = np.clip(np.floor(32.0 * 8.0 ** u[0]).astype(int), 32, 256)
+    xi_n = 0.1 + 0.89 * u[1]
+    ac = 10.0 ** (4.0 * u[2])
+    return [dict(n=int(n[m]), xi_n=float(xi_n[m]), col=float(ac[m])) for m in range(n_members)]
+
+
+def ensemble_member_2d(params) -> Arrays:
+    return orszag_tang(params["n"], xi_n=params["xi_n"], col=params["col"])
