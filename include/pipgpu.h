@@ -117,9 +117,9 @@ int pip_get_stream(pip_ctx *ctx, void **stream);
 int pip_set_tile(pip_ctx *ctx, int tx, int ty);
 /* Bit s-1 of stage_mask set: SSP-RK stage s takes the two fluids one after
  * the other in a tile of nvar_m planes (three workgroups per CU; tiles 16x15,
- * 32x7, 16x7; 2-D runs without ionisation/recombination and with explicit
- * constant or per-cell alpha, other runs keep the one-pass kernel); clear:
- * both fluids at once.  Bitwise the same results (autotuner). */
+ * 32x7, 16x7; 2-D runs with explicit constant or per-cell alpha, with or
+ * without set_IR; other runs keep the one-pass kernel); clear: both fluids
+ * at once.  Bitwise the same results (autotuner). */
 int pip_set_two_pass(pip_ctx *ctx, int stage_mask);
 /* Arithmetic of the stage kernels.  PIP_ARITH_EXACT (the default): the
  * reference's operations in the reference's order, no FMA contraction,

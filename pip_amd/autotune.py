@@ -43,7 +43,7 @@ def stage_key(cfg, arith: str = "exact") -> str:
 
 def two_pass_ok(cfg) -> bool:
     """Configurations the fluid-by-fluid stage kernel covers."""
-    return cfg.ndim == 2 and cfg.flag_ir == 0 and cfg.flag_col == 1 and cfg.flag_pip_imp == 0
+    return cfg.ndim == 2 and cfg.flag_col == 1 and cfg.flag_pip_imp == 0
 
 
 def tune_stage(make_solver, cfg, steps: int = 20, use_cache: bool = True, arith: str = "exact"):

@@ -506,7 +506,7 @@ struct TileGeom2 {
 // plasma's partial cell value waits out the y faces in the cell's own
 // output entries (read back by the same thread; in place, the stage base
 // there was read first) rather than in registers next to HLLD's.
-template <int FL, int NV, int TX, int TY, int NT, bool REPLAY>
+template <int FL, int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
 __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, double c0, double c1,
                                            double dt_sub, int i0, int j0, double* smem, bool valid,
                                            long long oc, int ti, int tj, double* un) {
@@ -529,6 +529,11 @@ __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, doub
       double u[NFL], v[NFL];
 #pragma unroll
       for (int n = 0; n < NFL; n++) u[n] = a.Uin[(OFF + n) * N + o];
+      if (IR) {  // set_IR of set_coefficients(.,1) rewrites E (Solver_rot.f90:111-113), per fluid
+        double pr, te;
+        if (FL == 0) getpr_mhd(u, pr, te, a.ph);
+        else getpr_hd(u, pr, te, a.ph);
+      }
       if (FL == 0) u2v_mhd<NV>(u, v, a.ph);
       else u2v_hd(u, v, a.ph);
 #pragma unroll
@@ -571,11 +576,17 @@ __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, doub
   __syncthreads();
 
   // ---- the SSP blend and the x divergence; the blend takes the stage
-  // input after source_PIP's cq2pv rewrite of this fluid (PIP_rot.f90:1328)
+  // input after set_IR's get_Pr rewrite (with IR) and source_PIP's cq2pv
+  // rewrite of this fluid (PIP_rot.f90:1328)
   if (valid) {
     double u[NFL], pv[8];
 #pragma unroll
     for (int n = 0; n < NFL; n++) u[n] = a.Uin[(OFF + n) * N + oc];
+    if (IR) {
+      double pr, te;
+      if (FL == 0) getpr_mhd(u, pr, te, a.ph);
+      else getpr_hd(u, pr, te, a.ph);
+    }
     if (FL == 0) cq2pv_mhd(u, pv, a.ph);
     else cq2pv_hd(u, pv, a.ph);
     const double ax = dt_sub / a.dx;
@@ -635,7 +646,7 @@ __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, doub
   }
 }
 
-template <int NV, int TX, int TY, int NT, bool REPLAY>
+template <int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
 __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
   extern __shared__ double smem[];
   constexpr int NVT = NV + 5;
@@ -661,9 +672,9 @@ __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
   const long long N = a.N;
   const long long oc = (long long)cj * a.ix + ci;
   double un[NVT];
-  fluid_pass<0, NV, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un);
+  fluid_pass<0, NV, IR, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un);
   __syncthreads();  // the neutral pass reuses the tile
-  fluid_pass<1, NV, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un + NV);
+  fluid_pass<1, NV, IR, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un + NV);
 
   RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
   int nanflag = 0;
@@ -676,10 +687,15 @@ __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
 #pragma unroll
     for (int n = 0; n < 5; n++) uh[n] = a.Uin[(NV + n) * N + oc];
     const double acc = a.ac ? a.ac[oc] : a.col;
+    double gr = 0.0, gi = 0.0, arb = 0.0;
+    if (IR) {  // set_IR (rewrites and rates) before source_PIP, as in stage_tile
+      set_ir_cell<NV>(um, uh, a.ph, a.irc, gr, gi);
+      if (a.ph.flag_ir_type == 0) arb = a.arb_heat[oc];
+    }
     double Sm[5], Sh[5], ion_pot;
     Phys phs = a.ph;
     phs.pip_imp = 0;
-    source_pip(um, uh, acc, 0.0, 0.0, 0.0, phs, a.irc.heat_fac, Sh, Sm, ion_pot);
+    source_pip(um, uh, acc, gr, gi, arb, phs, a.irc.heat_fac, Sh, Sm, ion_pot);
     if (a.ion_pot_out) a.ion_pot_out[oc] = ion_pot;
 #pragma unroll
     for (int n = 0; n < 5; n++) {
@@ -691,8 +707,8 @@ __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
       un[8] = un[8] * fac;
     }
     if (epi >= 1) {
-      if (epi >= 2) cfl_rewrite<NV, false>(un, un + NV, a);  // main.f90:44-47, its dt unused
-      r = cfl_round<NV, false>(un, un + NV, acc, ion_pot, a);
+      if (epi >= 2) cfl_rewrite<NV, IR>(un, un + NV, a);  // main.f90:44-47, its dt unused
+      r = cfl_round<NV, IR>(un, un + NV, acc, ion_pot, a);
 #pragma unroll
       for (int n = 0; n < NV; n++) nanflag |= !(un[n] * 0.0 == 0.0);
     }

@@ -200,11 +200,11 @@ static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   }
 }
 
-template <int NV, int TX, int TY, int NT, bool REPLAY>
+template <int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
 static int launch_stage2p_t(pip_ctx* c, const StageArgs& a) {
   constexpr size_t lds = sizeof(double) * (size_t)TileGeom2<NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage2p<NV, TX, TY, NT, REPLAY>;
+  auto kern = k_stage2p<NV, IR, TX, TY, NT, REPLAY>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
@@ -215,22 +215,23 @@ static int launch_stage2p_t(pip_ctx* c, const StageArgs& a) {
   return PIP_OK;
 }
 
-// the fluid-by-fluid stage kernel: 2-D, no IR, explicit constant/per-cell alpha
-template <int NV, bool REPLAY>
+// the fluid-by-fluid stage kernel: 2-D, explicit constant/per-cell alpha,
+// with or without set_IR (replays: without)
+template <int NV, bool IR, bool REPLAY>
 static int launch_stage2p(pip_ctx* c, const StageArgs& a) {
-  if (c->tx == 16 && c->ty == 15) return launch_stage2p_t<NV, 16, 15, 256, REPLAY>(c, a);
-  if (c->tx == 32 && c->ty == 7) return launch_stage2p_t<NV, 32, 7, 256, REPLAY>(c, a);
-  if (c->tx == 16 && c->ty == 7) return launch_stage2p_t<NV, 16, 7, 128, REPLAY>(c, a);
+  if (c->tx == 16 && c->ty == 15) return launch_stage2p_t<NV, IR, 16, 15, 256, REPLAY>(c, a);
+  if (c->tx == 32 && c->ty == 7) return launch_stage2p_t<NV, IR, 32, 7, 256, REPLAY>(c, a);
+  if (c->tx == 16 && c->ty == 7) return launch_stage2p_t<NV, IR, 16, 7, 128, REPLAY>(c, a);
   return fail(PIP_ERR_ARG, "the two-pass stage kernel takes tiles 16x15, 32x7 and 16x7");
 }
 
 template <int NDIM, int NV, bool IR>
 static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
   const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
-  if constexpr (NDIM == 2 && !IR) {
+  if constexpr (NDIM == 2) {
     if (((c->two_pass >> (a.istep - 1)) & 1) && !ext) {
-      if (!a.dtc) return launch_stage2p<NV, false>(c, a);
-      if constexpr (NV == 9) return launch_stage2p<9, true>(c, a);
+      if (!a.dtc) return launch_stage2p<NV, IR, false>(c, a);
+      if constexpr (NV == 9 && !IR) return launch_stage2p<9, false, true>(c, a);
     }
   }
   if (a.dtc) {  // frozen-dt replay kernels: the adjoint configuration only

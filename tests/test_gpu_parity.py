@@ -169,6 +169,31 @@ def test_two_pass_stage_bitwise_equal(tile, bnd):
     assert ref.scalars() == s.scalars()
 
 
+@pytest.mark.parametrize("tile", [(16, 15, 1, 3), (32, 7, 0, 2), (16, 7, 1, 1)])
+@pytest.mark.parametrize("ir_type", [0, 1])
+def test_two_pass_stage_with_ir_bitwise_equal(tile, ir_type):
+    """With set_IR case 3 (type 0: ion_pot / arb_heat; type 1) the
+    fluid-by-fluid stage kernel gives the one-pass kernel's result bit for
+    bit: Kelvin-Helmholtz 72^2 (x periodic, y mirror), 30 steps, and the
+    oracle's state after the same steps."""
+    say("case kelvin_helmholtz n=72 two-pass with IR", tile, "type", ir_type)
+    cfg, uh, um, ac = problems.kelvin_helmholtz(72)
+    cfg = PIPConfig(**{**cfg.__dict__, "flag_ir_type": ir_type})
+    ref = PIPSolver(cfg, uh, um, ac, tile=(16, 15, 1, 0))
+    ref.advance(nsteps=30)
+    s = PIPSolver(cfg, uh, um, ac, tile=tile)
+    s.advance(nsteps=30)
+    for a, b in zip(ref.state(), s.state()):
+        assert np.array_equal(a, b)
+    assert ref.scalars() == s.scalars()
+    o = Oracle(cfg, uh, um, ac)
+    assert o.run(nsteps=30) == 0
+    oh, om = o.state()
+    gh, gm = s.state()
+    assert np.array_equal(interior(om, cfg), interior(gm, cfg))
+    assert np.array_equal(interior(oh, cfg), interior(gh, cfg))
+
+
 def test_two_pass_stage_bitwise_at_full_size():
     say("case orszag_tang n=1024 two-pass vs one-pass, 3 steps")
     cfg, uh, um, ac = problems.orszag_tang(1024)
