@@ -37,7 +37,9 @@ def cost_estimate(cfg, U_h, U_m, tend: float) -> float:
     cf = np.sqrt(np.maximum(gm * p / rho + b2 / rho, 0.0))
     dt = cfg.safety * cfg.dx / np.max(v + cf)
     dt = min(dt, min(cfg.safety, 0.3) / max(np.max(rho * cfg.col), np.max(uh[0] * cfg.col)))
-    return cfg.nx * math.ceil(tend / dt)
+    if not (np.isfinite(dt) and dt > 0.0):
+        return float(cfg.nx)  # a state the first step's stop_sim ends: one step's work
+    return cfg.nx * min(math.ceil(tend / dt), 1 << 40)
 
 
 class Ensemble:

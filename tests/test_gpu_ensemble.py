@@ -206,3 +206,62 @@ def test_2d_ensemble_rejects_the_persistent_mode():
     with pytest.raises(PipError, match="2-D ensembles run in lockstep"):
         ens.run()
     ens.close()
+
+
+def _stop_members(twod):
+    """Members of which one plants a NaN in the magnetic field and one takes
+    a collision coefficient so large that dt falls below its floor
+    (IO_rot.f90:625-674); the others run normally."""
+    if twod:
+        mem = _members_2d()[:4]
+        cfg, uh, um = mem[1]
+        um = um.copy()
+        um[7, 9, 14] = np.nan
+        mem[1] = (cfg, uh, um)
+        cfg, uh, um, _ = problems.orszag_tang(24, col=1.0e14)
+        mem[3] = (cfg, uh, um)
+    else:
+        mem = _members(6, nx_cap=700)
+        cfg, uh, um = mem[1]
+        um = um.copy()
+        um[6, 0, 30] = np.nan
+        mem[1] = (cfg, uh, um)
+        p = dict(problems.ensemble_members(256, seed=7)[4], nx=300, col=1.0e13)
+        mem[4] = problems.ensemble_member(p)[:3]
+    return mem
+
+
+@pytest.mark.parametrize("mode,twod", [("persistent", False), ("lockstep", False), ("graph", False),
+                                       ("lockstep", True), ("graph", True)])
+def test_members_stop_like_their_plain_runs(mode, twod):
+    """A member whose state holds a NaN stops after its first step with the
+    NaN flag, one whose dt falls below the floor stops with the dt flag, as
+    their plain pip_advance runs do; the other members run on, bitwise equal
+    to their plain runs."""
+    print("case ensemble stop_sim members, mode", mode, "2-D" if twod else "1-D", flush=True)
+    mem = _stop_members(twod)
+    steps = 60 if twod else MAXS
+    ens = Ensemble(mem, tend=1.0e30 if twod else 0.02, max_steps=steps)
+    ens.set_mode(mode)
+    ens.run()
+    stops = set()
+    for k, (cfg, uh, um) in enumerate(mem):
+        s = PIPSolver(cfg, uh, um)
+        n, t = s.advance(nsteps=steps, tend=1.0e30 if twod else 0.02)
+        r = ens.result(k)
+        assert r["status"] == s.status(), (k, r, s.status())
+        assert r["nt"] == n, (k, r, n)
+        if not r["status"] & 2:
+            # (the dt of a state holding a NaN is not defined by the
+            # reference: only the stop and its step are compared)
+            assert r["time"] == t, (k, r, t)
+        if r["status"]:
+            stops.add(r["status"] & 3)
+        else:
+            eh, em = ens.state(k)
+            ph, pm = s.state()
+            i = (slice(None), slice(2, -2), slice(2, -2)) if twod else (slice(None), slice(None), slice(2, -2))
+            assert np.array_equal(em[i], pm[i]) and np.array_equal(eh[i], ph[i]), k
+        s.close()
+    ens.close()
+    assert stops == {1, 2}
