@@ -59,7 +59,8 @@ def pmc_profile_path(wl: str, n: int, arith: str = "exact") -> str:
 # the tests that hold each arithmetic to the oracle on the headline workload
 PARITY_TEST = {
     "exact": "tests/test_gpu_parity_full.py::test_kelvin_helmholtz_ir_2048_100_steps (bitwise, 100 steps at "
-             "2048^2) and ::test_kelvin_helmholtz_ir_384_1000_steps (bitwise, all 1000 steps at 384^2)",
+             "2048^2), ::test_kelvin_helmholtz_ir_384_1000_steps (bitwise, 1000 steps at 384^2) and "
+             "tests/test_gpu_parity_long.py (opt-in: bitwise, all 1000 steps at 2048^2)",
     "relaxed": "tests/test_gpu_parity_full.py::test_kelvin_helmholtz_ir_2048_100_steps (after 100 steps "
                "relative L1 <= 1e-10 on the order-one variables, <= 1e-5 on the perturbation-born ones; "
                "does not meet a per-variable 1e-12 bar) and tests/test_gpu_relaxed.py (shock positions)",
