@@ -7,7 +7,9 @@
 // One launch of k_stage performs one SSP-RK stage for both fluids on a
 // TX x TY tile of interior cells per workgroup:
 //   load  : (TX+4)x(TY+4) cells -> primitives in LDS (u2v_*, HLL_rot.f90:1136-1186)
-//   x     : (TX+1)*TY faces: minmod MUSCL + HLLD + HLLC -> flux tile in LDS
+//   x     : (TX+1)*TY faces: minmod MUSCL + HLLD, then minmod MUSCL + HLLC
+//           (one fluid at a time, so the neutral face states are not live
+//           while HLLD runs) -> flux tile in LDS
 //   cell  : source_PIP + SSP blend + x flux divergence (Solver_rot.f90:464-478)
 //   y     : TX*(TY+1) faces (permuted HLLD/HLLC) -> flux tile in LDS
 //   cell  : y flux divergence, add_source, Dedner psi damping, then (last
