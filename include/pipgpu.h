@@ -171,6 +171,14 @@ int pip_record_steps(pip_ctx *ctx, int n, double *hist);
  * step count have advanced by the n steps. */
 int pip_replay_steps(pip_ctx *ctx, int n, const double *hist, double *Un, double *U1, double *Vn,
                      double *V1);
+/* pip_record_steps that also keeps what pip_replay_steps would save: the
+ * stage inputs U_n, U1 of step k and their primitives at plane offset
+ * k*(nvar_m+5) of Un, U1, Vn, V1 (all required).  The same steps and bits as
+ * pip_record_steps; for the last checkpoint segment of a recorded run, whose
+ * replay the backward sweep can then skip.  Covers the configuration of
+ * pip_replay_steps's primitive saves in the exact arithmetic. */
+int pip_record_steps_saved(pip_ctx *ctx, int n, double *hist, double *Un, double *U1, double *Vn,
+                           double *V1);
 /* adjoint seed: add the ghost-cell entries of lam onto the interior cells
  * they image (boundary sign maps), then zero the ghosts */
 int pip_adj_seed(pip_ctx *ctx, double *lam);

@@ -99,6 +99,7 @@ _SIGS = {
     "pip_set_ac_device": [_V, _V],
     "pip_record_steps": [_V, _I, _V],
     "pip_replay_steps": [_V, _I, _V, _V, _V, _V, _V],
+    "pip_record_steps_saved": [_V, _I, _V, _V, _V, _V, _V],
     "pip_adj_seed": [_V, _V],
     "pip_adj_step": [_V, _V, _V, _V, _V, _V, _V, _V],
     "pip_adj_step_noalloc": [_V, _V, _V, _V, _V, _V, _V, _V],

@@ -9,12 +9,14 @@ collision coefficient field ``ac`` (jx, ix).  ``run(U0, ac)`` returns the
 final state and is differentiable with respect to both inputs:
 
   forward   records dt/cmax of every step on the device and keeps the state
-            at every ``ckpt_every``-th step (step checkpointing);
+            at every ``ckpt_every``-th step (step checkpointing); the last
+            segment also keeps its stage inputs (what its replay would save);
   backward  for each segment, newest first: restore its checkpoint, replay
             the segment with the recorded dt (bitwise the forward run)
-            saving the stage inputs, then pull the adjoint back through
-            its steps (hand-written reverse-mode kernels: stage update,
-            HLLD/HLLC fluxes, minmod reconstruction, collision source).
+            saving the stage inputs (the last segment's are still there
+            from the forward), then pull the adjoint back through its steps
+            (hand-written reverse-mode kernels: stage update, HLLD/HLLC
+            fluxes, minmod reconstruction, collision source).
 
 The gradient is that of the discrete map with the recorded time-step
 sequence held fixed (dt and the GLM speed cmax are treated as constants);
@@ -69,6 +71,9 @@ class TwoFluidRun:
         self._seg_U1 = None
         self._seg_Vn = None
         self._seg_V1 = None
+        # the forward run whose last segment the buffers hold (None: none)
+        self._seg_run = None
+        self.last_run = None
         # the replay also saves the stage inputs' primitives (twice the segment
         # memory, no primitive pass in the adjoint step)
         self.save_primitives = bool(save_primitives)
@@ -114,6 +119,17 @@ class TwoFluidRun:
         if U0.device != self.device or ac.device != self.device:
             raise ValueError(f"inputs must live on {self.device}")
 
+    def _segment_buffers(self):
+        """Stage inputs (and their primitives) of one segment, 4 states per
+        step, allocated on first use."""
+        mmax = max((m for _, m in self._segments()), default=0)
+        if mmax and (self._seg_Un is None or self._seg_Un.shape[0] < mmax):
+            self._seg_Un = self._empty_state(mmax)
+            self._seg_U1 = self._empty_state(mmax)
+            if self.save_primitives:
+                self._seg_Vn = self._empty_state(mmax)
+                self._seg_V1 = self._empty_state(mmax)
+
     def _empty_state(self, n=None):
         c = self.cfg
         shape = (self.nvt, c.jx, c.ix) if n is None else (n, self.nvt, c.jx, c.ix)
@@ -123,15 +139,27 @@ class TwoFluidRun:
     def forward_record(self, U0: torch.Tensor, ac: torch.Tensor):
         """Forward run recording dt/cmax; returns (U_T, checkpoints, history),
         history being this run's own copy of the recorded (dt, cmax) pairs
-        (the runner's buffer is overwritten by the next forward)."""
+        (the runner's buffer is overwritten by the next forward).  The last
+        segment runs through pip_record_steps_saved, which leaves its stage
+        inputs in the segment buffers; ``last_run`` names this run for
+        backward() until another forward or a replay overwrites them."""
         self._start(U0, ac)
         L = lib()
         ckpts = []
-        for k0, m in self._segments():
+        segs = self._segments()
+        self._seg_run = None
+        for s_, (k0, m) in enumerate(segs):
             ck = self._empty_state()
             check(L.pip_get_state_device(self.h, _ptr(ck)))
             ckpts.append(ck)
-            check(L.pip_record_steps(self.h, m, _ptr(self.hist) + 16 * k0))
+            if s_ == len(segs) - 1 and self.save_primitives:
+                self._segment_buffers()
+                check(L.pip_record_steps_saved(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
+                                               _ptr(self._seg_U1), _ptr(self._seg_Vn), _ptr(self._seg_V1)))
+                self._seg_run = object()
+            else:
+                check(L.pip_record_steps(self.h, m, _ptr(self.hist) + 16 * k0))
+        self.last_run = self._seg_run
         out = self._empty_state()
         check(L.pip_get_state_device(self.h, _ptr(out)))
         self._lib_done()
@@ -152,34 +180,32 @@ class TwoFluidRun:
         self._lib_done()
         return out
 
-    def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor, hist: torch.Tensor):
+    def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor, hist: torch.Tensor, run=None):
         """dJ/dU0, dJ/dac from dJ/dU_T by checkpointed reverse sweeps over the
-        forward run that produced ckpts and hist."""
+        forward run that produced ckpts and hist (``run``: its ``last_run``;
+        when the segment buffers still hold that run's last segment, its
+        replay is skipped)."""
         if hist.numel() < 2 * self.nsteps or len(ckpts) != len(self._segments()):
             raise ValueError("checkpoints and history do not belong to a run of this runner")
         L = lib()
         lam = gU.detach().to(torch.float64).contiguous().clone()
         lam_ac = torch.zeros_like(ac)
         segs = self._segments()
-        mmax = max((m for _, m in segs), default=0)
-        if mmax and (self._seg_Un is None or self._seg_Un.shape[0] < mmax):
-            # stage inputs and their primitives for one segment (4 states per step)
-            self._seg_Un = self._empty_state(mmax)
-            self._seg_U1 = self._empty_state(mmax)
-            if self.save_primitives:
-                self._seg_Vn = self._empty_state(mmax)
-                self._seg_V1 = self._empty_state(mmax)
+        self._segment_buffers()
+        kept = run is not None and run is self._seg_run
         self._torch_done()
         check(L.pip_set_ac_device(self.h, _ptr(ac)))
         check(L.pip_adj_seed(self.h, _ptr(lam)))
         sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
         sv = self.save_primitives
         self._use_tile(self.replay_tile)
-        for (k0, m), ck in zip(reversed(segs), reversed(ckpts)):
-            check(L.pip_set_state_device(self.h, _ptr(ck)))
-            check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(self._seg_Un),
-                                     _ptr(self._seg_U1), _ptr(self._seg_Vn) if sv else None,
-                                     _ptr(self._seg_V1) if sv else None))
+        for s_, ((k0, m), ck) in enumerate(zip(reversed(segs), reversed(ckpts))):
+            if not (s_ == 0 and kept):
+                self._seg_run = None
+                check(L.pip_set_state_device(self.h, _ptr(ck)))
+                check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(self._seg_Un),
+                                         _ptr(self._seg_U1), _ptr(self._seg_Vn) if sv else None,
+                                         _ptr(self._seg_V1) if sv else None))
             for k in reversed(range(m)):
                 check(L.pip_adj_step(self.h, _ptr(self._seg_Un) + sb * k, _ptr(self._seg_U1) + sb * k,
                                      _ptr(self._seg_Vn) + sb * k if sv else None,
@@ -209,6 +235,7 @@ class _Evolve(torch.autograd.Function):
         out, ckpts, hist = runner.forward_record(U0c, acc)
         ctx.runner = runner
         ctx.ckpts = ckpts
+        ctx.run = runner.last_run
         # this call's own dt/cmax sequence: a later forward on the same
         # runner overwrites runner.hist, not this copy
         ctx.save_for_backward(acc, hist)
@@ -217,7 +244,7 @@ class _Evolve(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gU):
         acc, hist = ctx.saved_tensors
-        lam, lam_ac = ctx.runner.backward(gU, ctx.ckpts, acc, hist)
+        lam, lam_ac = ctx.runner.backward(gU, ctx.ckpts, acc, hist, run=ctx.run)
         return lam, lam_ac, None
 
 

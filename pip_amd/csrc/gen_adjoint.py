@@ -99,6 +99,16 @@ class Block:
         self.items.append(("let", s, (cond, sp.sympify(a), sp.sympify(b)), "sel"))
         return s
 
+    def switch(self, name, x, eps):
+        """name = (|x| + x) / max(2 x, eps) for a symbol x (HLL_rot.f90:679-681):
+        1 for 2 x > eps, 0 for x <= 0, 2 x / eps in between.  Its derivative
+        is 2 / eps inside that band and exactly 0 outside; the general rule
+        for the quotient would form the zero as the difference of two terms
+        of size 1 / x, whose rounding leaves a residual that grows as x -> 0."""
+        s = self.g.sym(name)
+        self.items.append(("let", s, (sp.sympify(x), float(eps)), "switch"))
+        return s
+
     def const(self, name, code):
         """A value with zero derivative (e.g. sign), given as C code."""
         s = self.g.sym(name)
@@ -186,12 +196,15 @@ class Gen:
                     out.append(f"{pad}{s} = ({cc(a)} > {cc(b)}) ? {cc(a)} : {cc(b)};")
                 elif kind == "sel":
                     out.append(f"{pad}{s} = ({e[0]}) ? {cc(e[1])} : {cc(e[2])};")
+                elif kind == "switch":
+                    x, eps = cc(e[0]), e[1]
+                    out.append(f"{pad}{s} = (2.0 * {x} > {eps!r}) ? 1.0 : (({x} > 0.0) ? {x} * {2.0 / eps!r} : 0.0);")
                 elif kind == "div":
                     out.append(f"{pad}{s} = PIP_VDIV({cc(e[0])}, {cc(e[1])});")
                 elif kind == "sqrt":  # constants fold through sqrt
                     out.append(f"{pad}{s} = {'sqrt' if e.is_Number else 'PIP_VSQRT'}({cc(e)});")
                 else:
-                    out.append(f"{pad}{s} = {cc(e)};")
+                    out.append(f"{pad}{{ PIP_FWD_EXACT {s} = {cc(e)}; }}")
             else:
                 pass  # branches handled by _emit
 
@@ -211,6 +224,11 @@ class Gen:
                     out.append(f"{pad}a_{x} += ({cond}) ? {ab} : 0.0;")
                 if y in known:
                     out.append(f"{pad}a_{y} += ({cond}) ? 0.0 : {ab};")
+                continue
+            if kind == "switch":
+                x, eps = e
+                if x in set(self.syms.values()):
+                    out.append(f"{pad}a_{x} += ({cc(x)} > 0.0 && 2.0 * {cc(x)} <= {eps!r}) ? {ab} * {2.0 / eps!r} : 0.0;")
                 continue
             if kind == "div":
                 n, d = e
@@ -344,9 +362,9 @@ def gen_hllc():
             B.let("str4", ((sr - vnr) * ur[4] - pr * vnr + prm * sm) * ir)]
     # smooth switches (HLL_rot.f90:679-681): derivative inside the epsi band
     # is kept; outside it they are constants 0/1
-    thl = B.let("thl", (sp.Abs(sl) + sl) / sp.Max(2 * sl, epsi))
-    thm = B.let("thm", (sp.Abs(sm) + sm) / sp.Max(2 * sm, epsi))
-    thr = B.let("thr", 1 - (sp.Abs(sr) + sr) / sp.Max(2 * sr, epsi))
+    thl = B.switch("thl", sl, epsi)
+    thm = B.switch("thm", sm, epsi)
+    thr = B.let("thr", 1 - B.switch("swr", sr, epsi))
     for k in range(5):
         B.let(f"F{k}", thr * fr[k] + thl * fl[k] + (1 - thl - thr) *
               (thm * (fl[k] + sl * (stl[k] - ul[k])) + (1 - thm) * (fr[k] + sr * (st_r[k] - ur[k]))))
@@ -544,7 +562,7 @@ def gen_source():
 RECOMPUTE = os.environ.get("PIP_ADJ_RECOMPUTE", "1") == "1"
 
 
-def main():
+def main(path=None):
     parts = [
         "// pip_adj_gen.h -- GENERATED by gen_adjoint.py; do not edit by hand.\n"
         "// Vector-Jacobian products of the HLLC flux (HLL_rot.f90:615-687), the\n"
@@ -582,16 +600,18 @@ def main():
         "#define PIP_VDIV(n, d) pip_vdiv((n), (d))\n"
         "#define PIP_VRDIV(n, d) ((n) * pip_vrcp(d))\n"
         "#define PIP_VJP_CONTRACT _Pragma(\"clang fp contract(fast)\")\n"
+        "#define PIP_FWD_EXACT _Pragma(\"clang fp contract(off)\")\n"
         "#else\n"
         "#define PIP_VDIV(n, d) ((n) / (d))\n"
         "#define PIP_VRDIV(n, d) ((n) / (d))\n"
         "#define PIP_VJP_CONTRACT\n"
+        "#define PIP_FWD_EXACT\n"
         "#endif\n"
         "#define PIP_VSQRT(x) qsqrt(x)\n\n"
         "namespace pip {\n\n",
         gen_hllc(), "\n", gen_hlld(), "\n", gen_source(), "\n}  // namespace pip\n",
     ]
-    path = os.path.join(HERE, "pip_adj_gen.h")
+    path = path or os.path.join(HERE, "pip_adj_gen.h")
     with open(path, "w") as f:
         f.write("".join(parts))
     print("wrote", path)

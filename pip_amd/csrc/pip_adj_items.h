@@ -27,6 +27,12 @@
 //   per cell: recon^T -- gather those planes from the faces whose stencils
 //             hold the cell and its ghost images (periodic / mirror sign
 //             maps) -- then u2v^T, added to the pointwise part in lam_in.
+// The host checks run these items one after another.  The HIP kernels run
+// the plasma's face items over every face and the neutrals' only over the
+// faces next to the boundary (adj_edge_face_item); the gather kernel
+// computes the other neutral face adjoints of its tile itself (adj_face_vjp)
+// and keeps them in shared memory.  Every value is formed by the same
+// operations either way.
 #pragma once
 
 #include "pip_adj_gen.h"
@@ -175,20 +181,16 @@ struct alignas(16) FacePair {
   double l, r;
 };
 
-// One thread per face and fluid (FL 0: plasma, HLLD with GLM; FL 1:
-// neutrals, HLLC); R = the cell on the + side of the face.
-// dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
-// Writes the pair plane (dir * NVT + n) at R: the adjoints of the left and
-// right face states of variable n, density floor applied.
-template <int FL>
-PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
-  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+// The adjoints of the left and right states of the face with + cell
+// (fi, fj) in direction dir for the fluid FL (0: plasma, HLLD with GLM; 1:
+// neutrals, HLLC), density floor applied, into lvl / lvr (the fluid's
+// variables in the x-sweep order); the face's packed routing code goes to
+// code_out before the flux VJP (nothing of the stencil stays live across it).
+template <int FL, class CodeOut>
+PIP_HD void adj_face_vjp(const AdjArgs& a, int dir, int fi, int fj, double* lvl, double* lvr, CodeOut code_out) {
+  constexpr int NV = ADJ_NV;
   constexpr int NF = FL ? 5 : NV;   // variables of this fluid
   constexpr int OFF = FL ? NV : 0;  // first plane of this fluid
-  const int wx = (dir == 0) ? a.nx + 1 : a.nx;
-  const int wy = (dir == 0) ? a.ny : a.ny + 1;
-  if (t >= (long long)wx * wy) return;
-  const int fi = (int)(t % wx) + 2, fj = (int)(t / wx) + 2;
   const long long N = a.N;
   const long long st = (dir == 0) ? 1 : a.ix;
   const long long oR = (long long)fj * a.ix + fi;
@@ -211,8 +213,6 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
     const double mL = intL ? adj_mu(a, s, oL, psi_fac) : 0.0;
     lamF[n] = coef * (mR - mL);
   }
-  // the minmod branch codes go out first (nothing of the stencil stays live
-  // across the flux VJP)
   {
     unsigned long long codes = 0ull;
 #pragma unroll
@@ -223,25 +223,62 @@ PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
       const unsigned cr = recon_code(p[-st], p[0], p[st]);
       codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
     }
-    adj_code_pairs(a, dir)[oR].c[FL] = codes;
+    code_out(codes);
   }
   // density floor of get_Side (HLL_rot.f90:60-61)
   const bool flL = !(vl[0] > a.ph.ro_lim), flR = !(vr[0] > a.ph.ro_lim);
   vl[0] = dmax(vl[0], a.ph.ro_lim);
   vr[0] = dmax(vr[0], a.ph.ro_lim);
-  double lvl[NF], lvr[NF];
 #pragma unroll
   for (int n = 0; n < NF; n++) { lvl[n] = 0.0; lvr[n] = 0.0; }
   if (FL == 0) hlld9_flux_vjp(vl, vr, a.ph.gm, a.dtc[1], lamF, lvl, lvr);
   else hllc_flux_vjp(vl, vr, a.ph.gm, lamF, lvl, lvr);
   if (flL) lvl[0] = 0.0;
   if (flR) lvr[0] = 0.0;
+}
+
+// One thread per face and fluid; R = the cell on the + side of the face.
+// dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
+// Writes the pair plane (dir * NVT + n) at R: the adjoints of the left and
+// right face states of variable n, and the face's routing code.
+template <int FL>
+PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
+  constexpr int NVT = ADJ_NVT;
+  constexpr int NF = FL ? 5 : ADJ_NV;
+  constexpr int OFF = FL ? ADJ_NV : 0;
+  const int wx = (dir == 0) ? a.nx + 1 : a.nx;
+  const int wy = (dir == 0) ? a.ny : a.ny + 1;
+  if (t >= (long long)wx * wy) return;
+  const int fi = (int)(t % wx) + 2, fj = (int)(t / wx) + 2;
+  const long long N = a.N;
+  const long long oR = (long long)fj * a.ix + fi;
+  double lvl[NF], lvr[NF];
+  adj_face_vjp<FL>(a, dir, fi, fj, lvl, lvr,
+                   [&](unsigned long long c) { adj_code_pairs(a, dir)[oR].c[FL] = c; });
   FacePair* P = reinterpret_cast<FacePair*>(a.P) + (long long)dir * NVT * N;
 #pragma unroll
   for (int n = 0; n < NF; n++) {
     const int s = adj_perm(OFF + n, dir);
     P[s * N + oR] = FacePair{lvl[n], lvr[n]};
   }
+}
+
+// The neutral faces the gather reads at the ghost images of boundary cells:
+// x faces with + cell i in {2, 3, nx+1, nx+2} (rows 2..ny+1) and y faces
+// with + cell j in {2, 3, ny+1, ny+2} (columns 2..nx+1); item t of
+// adj_edge_faces(nx, ny).  Writes them like adj_face_item<1>.
+PIP_HD long long adj_edge_faces(int nx, int ny) { return 4LL * ny + 4LL * nx; }
+PIP_HD void adj_edge_face_item(const AdjArgs& a, long long t) {
+  if (t >= adj_edge_faces(a.nx, a.ny)) return;
+  const int dir = (t < 4LL * a.ny) ? 0 : 1;
+  if (dir) t -= 4LL * a.ny;
+  const int k = (int)(t % 4), r = (int)(t / 4);  // k-th edge face of row / column r
+  const int w = dir ? a.ny : a.nx;
+  const int c = (k < 2) ? 2 + k : w + k - 1;     // 2, 3, w+1, w+2
+  if (k >= 2 && c <= 3) return;                  // w <= 2: already among 2, 3
+  const int fi = dir ? r + 2 : c, fj = dir ? c : r + 2;
+  const int wx = (dir == 0) ? a.nx + 1 : a.nx;
+  adj_face_item<1>(a, dir, (long long)(fj - 2) * wx + (fi - 2));
 }
 
 // Left / right face-state weights of the cell at position q (0, 1, 2 =
@@ -283,10 +320,29 @@ PIP_HD void adj_gather_fluid(const AdjArgs& a, int dir, int n0, long long o2, lo
   }
 }
 
+// The pointwise part and the transpose of u2v at an interior cell, after the
+// face contributions lv: lam_in = pointwise + u2v^T lv.
+PIP_HD void adj_cell_finish(const AdjArgs& a, long long k, double* lv) {
+  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  const long long N = a.N;
+  double u[NVT], lu[NVT], t[NVT];
+#pragma unroll
+  for (int n = 0; n < NVT; n++) u[n] = a.Uin[n * N + k];
+  adj_point_part(a, k, u, u + NV, t);
+#pragma unroll
+  for (int n = 0; n < NVT; n++) lu[n] = 0.0;
+  u2v_vjp<NV>(u, lv, a.ph, lu);
+  u2v_vjp_hd(u + NV, lv + NV, a.ph, lu + NV);
+#pragma unroll
+  for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = t[n] + lu[n];
+}
+
 // Sum of the face contributions to the primitives of the cell at (i, j),
-// with the sign map `mask` of a ghost image applied.  A face with + cell r
+// with the sign map `mask` of a ghost image applied (the plasma's only when
+// NEUTRAL is false).  A face with + cell r
 // (stride st) takes V_L from cells r-2, r-1, r and V_R from r-1, r, r+1,
 // with the weights of the recorded minmod branches.
+template <bool NEUTRAL = true>
 PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double* lv) {
   constexpr int NV = ADJ_NV;
   const long long o = (long long)j * a.ix + i;
@@ -305,7 +361,8 @@ PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double*
     const CodePair* C = adj_code_pairs(a, dir);
     const CodePair k2 = C[o2], k1 = C[o1], k0 = C[o0], km = C[om];
     adj_gather_fluid<NV>(a, dir, 0, o2, o1, o0, om, v2, v1, v0, vm, k2.c[0], k1.c[0], k0.c[0], km.c[0], mask, lv);
-    adj_gather_fluid<5>(a, dir, NV, o2, o1, o0, om, v2, v1, v0, vm, k2.c[1], k1.c[1], k0.c[1], km.c[1], mask, lv);
+    if (NEUTRAL)
+      adj_gather_fluid<5>(a, dir, NV, o2, o1, o0, om, v2, v1, v0, vm, k2.c[1], k1.c[1], k0.c[1], km.c[1], mask, lv);
   }
 }
 
@@ -313,27 +370,17 @@ PIP_HD void adj_gather_at(const AdjArgs& a, int i, int j, unsigned mask, double*
 // ghost images, then the transpose of u2v (HLL_rot.f90:1136-1186) with the
 // forward point's floors; lam_in = pointwise part + that.
 PIP_HD void adj_gather_item(const AdjArgs& a, long long k) {
-  constexpr int NV = ADJ_NV, NVT = ADJ_NVT;
+  constexpr int NVT = ADJ_NVT;
   if (k >= a.N) return;
   const int ci = (int)(k % a.ix), cj = (int)(k / a.ix);
   if (!adj_interior(a, ci, cj)) return;
-  const long long N = a.N;
   double lv[NVT];
 #pragma unroll
   for (int n = 0; n < NVT; n++) lv[n] = 0.0;
   adj_gather_at(a, ci, cj, 0u, lv);
   for_each_image(ci, cj, a.ix, a.jx, a.bnd,
                  [&](int xi, int yj, unsigned m) { adj_gather_at(a, xi, yj, m, lv); });
-  double u[NVT], lu[NVT], t[NVT];
-#pragma unroll
-  for (int n = 0; n < NVT; n++) u[n] = a.Uin[n * N + k];
-  adj_point_part(a, k, u, u + NV, t);
-#pragma unroll
-  for (int n = 0; n < NVT; n++) lu[n] = 0.0;
-  u2v_vjp<NV>(u, lv, a.ph, lu);
-  u2v_vjp_hd(u + NV, lv + NV, a.ph, lu + NV);
-#pragma unroll
-  for (int n = 0; n < NVT; n++) a.lam_in[n * N + k] = t[n] + lu[n];
+  adj_cell_finish(a, k, lv);
 }
 
 // Adjoint seed on the ghost cells of a state: add each ghost's value (with

@@ -45,6 +45,13 @@ struct DevScalars {
   int pad[4];
 };
 
+// Stage kernel modes: a step of the time loop; a step of a frozen-dt replay
+// ({dt, cmax} from StageArgs::dtc, always active, never the loop's last
+// step, no dt reduction) that also writes the primitives of its input to
+// Vout; a step of the time loop that writes them as well (the last segment of
+// an adjoint's recording run).
+enum { STAGE_PLAIN = 0, STAGE_REPLAY = 1, STAGE_SAVE = 2 };
+
 // Boundary description: per side type (1 periodic, 2 mirror, 10 free) and
 // sign masks (bit n set -> variable n flips sign), Boundary_rot.f90:239-320.
 struct StageArgs {
@@ -56,8 +63,8 @@ struct StageArgs {
   double* U0out;          // implicit collisions: updated stage base for the next stage (nullable)
   const double* arb_heat; // IR type 0
   double* ion_pot_out;    // nullable: ion_pot of this stage
-  double* Vout;           // replay kernels, nullable: primitives of Uin at the tile's cells
-                          // and their ghost images
+  double* Vout;           // replay and saving kernels, nullable: primitives of Uin at the
+                          // tile's cells and their ghost images
   const double* dtc;      // replay kernels: frozen {dt, cmax} of the replayed step
   DevScalars* sc;
   int ix, jx, nx, ny;
@@ -203,7 +210,7 @@ struct TileGeom {
 // face phase in a single pass of the workgroup (e.g. 16x15 on 256 threads).
 // Returns whether the calling thread owns a cell; r / nanflag receive the
 // cell's dt terms when epi >= 1.
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY = false>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, int MODE = STAGE_PLAIN>
 __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double cmax, int epi, int i0,
                                            int j0, double* smem, RoundOut& r, int& nanflag) {
   using G = TileGeom<NDIM, NV, TX, TY>;
@@ -250,7 +257,8 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
       for (int n = 0; n < NV; n++) V[(n * HY + hy) * HX + hx] = vm[n];
 #pragma unroll
       for (int n = 0; n < 5; n++) V[((NV + n) * HY + hy) * HX + hx] = vh[n];
-      if (REPLAY && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 && (NDIM == 1 || (hy >= 2 && hy < TY + 2 && gj < a.ny + 2))) {
+      if (MODE != STAGE_PLAIN && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 &&
+          (NDIM == 1 || (hy >= 2 && hy < TY + 2 && gj < a.ny + 2))) {
         // a cell of this tile: its primitives and their ghost images (u2v
         // commutes with the boundary sign maps: v = m / rho, the pressure
         // takes squares)
@@ -453,12 +461,11 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, double dt, double
 // ------------------------------------------------------------ stage kernel
 // One workgroup per tile.  The look-ahead round of the fused step (epi=2)
 // is skipped on the loop's last step (sc->last), as main.f90 ends there.
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, int MODE>
 __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   extern __shared__ double smem[];
   DevScalars* sc = a.sc;
-  // REPLAY: a step of a frozen-dt replay ({dt, cmax} from a.dtc, always
-  // active, never the loop's last step, no dt reduction)
+  constexpr bool REPLAY = (MODE == STAGE_REPLAY);
   if (!REPLAY && !sc->active) return;
   const double dt = REPLAY ? a.dtc[0] : sc->dt;
   const double cmax = REPLAY ? a.dtc[1] : sc->cmax;
@@ -476,7 +483,7 @@ __global__ void __launch_bounds__(NT, 2) k_stage(StageArgs a) {
   const int j0 = (bid / ntx) * TY;
   RoundOut r;
   int nanflag;
-  bool valid = stage_tile<NDIM, NV, IR, EXT, TX, TY, NT, REPLAY>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
+  bool valid = stage_tile<NDIM, NV, IR, EXT, TX, TY, NT, MODE>(a, dt, cmax, epi, i0, j0, smem, r, nanflag);
   if (!REPLAY && a.epi >= 1) {
     __syncthreads();
     block_reduce_store<NT>(r, nanflag, valid, sc, smem);
@@ -508,7 +515,7 @@ struct TileGeom2 {
 // plasma's partial cell value waits out the y faces in the cell's own
 // output entries (read back by the same thread; in place, the stage base
 // there was read first) rather than in registers next to HLLD's.
-template <int FL, int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
+template <int FL, int NV, bool IR, int TX, int TY, int NT, int MODE>
 __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, double c0, double c1,
                                            double dt_sub, int i0, int j0, double* smem, bool valid,
                                            long long oc, int ti, int tj, double* un) {
@@ -540,7 +547,7 @@ __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, doub
       else u2v_hd(u, v, a.ph);
 #pragma unroll
       for (int n = 0; n < NFL; n++) V[(n * HY + hy) * HX + hx] = v[n];
-      if (REPLAY && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 && hy >= 2 && hy < TY + 2 &&
+      if (MODE != STAGE_PLAIN && a.Vout && hx >= 2 && hx < TX + 2 && gi < a.nx + 2 && hy >= 2 && hy < TY + 2 &&
           gj < a.ny + 2) {
         // a cell of this tile: its primitives and their ghost images
 #pragma unroll
@@ -648,11 +655,12 @@ __device__ __forceinline__ void fluid_pass(const StageArgs& a, double cmax, doub
   }
 }
 
-template <int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
+template <int NV, bool IR, int TX, int TY, int NT, int MODE>
 __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
   extern __shared__ double smem[];
   constexpr int NVT = NV + 5;
   DevScalars* sc = a.sc;
+  constexpr bool REPLAY = (MODE == STAGE_REPLAY);
   if (!REPLAY && !sc->active) return;
   const double dt = REPLAY ? a.dtc[0] : sc->dt;
   const double cmax = REPLAY ? a.dtc[1] : sc->cmax;
@@ -674,9 +682,9 @@ __global__ void __launch_bounds__(NT, 3) k_stage2p(StageArgs a) {
   const long long N = a.N;
   const long long oc = (long long)cj * a.ix + ci;
   double un[NVT];
-  fluid_pass<0, NV, IR, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un);
+  fluid_pass<0, NV, IR, TX, TY, NT, MODE>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un);
   __syncthreads();  // the neutral pass reuses the tile
-  fluid_pass<1, NV, IR, TX, TY, NT, REPLAY>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un + NV);
+  fluid_pass<1, NV, IR, TX, TY, NT, MODE>(a, cmax, c0, c1, dt_sub, i0, j0, smem, valid, oc, ti, tj, un + NV);
 
   RoundOut r = {1.0e300, 0.0, 0.0, 0.0};
   int nanflag = 0;

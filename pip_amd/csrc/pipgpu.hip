@@ -167,11 +167,11 @@ static StageArgs stage_args(pip_ctx* c, int istep, const double* Uin, const doub
   return a;
 }
 
-template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, bool REPLAY>
+template <int NDIM, int NV, bool IR, bool EXT, int TX, int TY, int NT, int MODE>
 static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   constexpr size_t lds = sizeof(double) * (size_t)TileGeom<NDIM, NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage<NDIM, NV, IR, EXT, TX, TY, NT, REPLAY>;
+  auto kern = k_stage<NDIM, NV, IR, EXT, TX, TY, NT, MODE>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
@@ -183,28 +183,28 @@ static int launch_stage_t(pip_ctx* c, const StageArgs& a) {
   return PIP_OK;
 }
 
-template <int NDIM, int NV, bool IR, bool EXT, bool REPLAY = false>
+template <int NDIM, int NV, bool IR, bool EXT, int MODE = STAGE_PLAIN>
 static int launch_stage_tile(pip_ctx* c, const StageArgs& a) {
   if constexpr (NDIM == 1) {
-    return launch_stage_t<1, NV, IR, EXT, 255, 1, 256, REPLAY>(c, a);
+    return launch_stage_t<1, NV, IR, EXT, 255, 1, 256, MODE>(c, a);
   } else {
     // tile shapes (cells) and workgroup sizes; the single-pass shapes keep
     // every face phase within one pass of the workgroup
-    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 16, 15, 256, REPLAY>(c, a);
-    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 32, 7, 256, REPLAY>(c, a);
-    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 32, 15, 512, REPLAY>(c, a);
-    if (c->tx == 64 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 64, 7, 512, REPLAY>(c, a);
-    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, EXT, 32, 8, 256, REPLAY>(c, a);
-    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 16, 7, 128, REPLAY>(c, a);
+    if (c->tx == 16 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 16, 15, 256, MODE>(c, a);
+    if (c->tx == 32 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 32, 7, 256, MODE>(c, a);
+    if (c->tx == 32 && c->ty == 15) return launch_stage_t<2, NV, IR, EXT, 32, 15, 512, MODE>(c, a);
+    if (c->tx == 64 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 64, 7, 512, MODE>(c, a);
+    if (c->tx == 32 && c->ty == 8) return launch_stage_t<2, NV, IR, EXT, 32, 8, 256, MODE>(c, a);
+    if (c->tx == 16 && c->ty == 7) return launch_stage_t<2, NV, IR, EXT, 16, 7, 128, MODE>(c, a);
     return fail(PIP_ERR_ARG, "unsupported tile shape");
   }
 }
 
-template <int NV, bool IR, int TX, int TY, int NT, bool REPLAY>
+template <int NV, bool IR, int TX, int TY, int NT, int MODE>
 static int launch_stage2p_t(pip_ctx* c, const StageArgs& a) {
   constexpr size_t lds = sizeof(double) * (size_t)TileGeom2<NV, TX, TY>::LDS_DOUBLES;
   static bool attr_done = false;
-  auto kern = k_stage2p<NV, IR, TX, TY, NT, REPLAY>;
+  auto kern = k_stage2p<NV, IR, TX, TY, NT, MODE>;
   if (!attr_done) {
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_done = true;
@@ -217,11 +217,11 @@ static int launch_stage2p_t(pip_ctx* c, const StageArgs& a) {
 
 // the fluid-by-fluid stage kernel: 2-D, explicit constant/per-cell alpha,
 // with or without set_IR (replays: without)
-template <int NV, bool IR, bool REPLAY>
+template <int NV, bool IR, int MODE>
 static int launch_stage2p(pip_ctx* c, const StageArgs& a) {
-  if (c->tx == 16 && c->ty == 15) return launch_stage2p_t<NV, IR, 16, 15, 256, REPLAY>(c, a);
-  if (c->tx == 32 && c->ty == 7) return launch_stage2p_t<NV, IR, 32, 7, 256, REPLAY>(c, a);
-  if (c->tx == 16 && c->ty == 7) return launch_stage2p_t<NV, IR, 16, 7, 128, REPLAY>(c, a);
+  if (c->tx == 16 && c->ty == 15) return launch_stage2p_t<NV, IR, 16, 15, 256, MODE>(c, a);
+  if (c->tx == 32 && c->ty == 7) return launch_stage2p_t<NV, IR, 32, 7, 256, MODE>(c, a);
+  if (c->tx == 16 && c->ty == 7) return launch_stage2p_t<NV, IR, 16, 7, 128, MODE>(c, a);
   return fail(PIP_ERR_ARG, "the two-pass stage kernel takes tiles 16x15, 32x7 and 16x7");
 }
 
@@ -230,15 +230,18 @@ static int launch_stage_ext(pip_ctx* c, const StageArgs& a) {
   const bool ext = col_type(c->ph) >= 1 || c->cfg.flag_pip_imp == 1;
   if constexpr (NDIM == 2) {
     if (((c->two_pass >> (a.istep - 1)) & 1) && !ext) {
-      if (!a.dtc) return launch_stage2p<NV, IR, false>(c, a);
-      if constexpr (NV == 9 && !IR) return launch_stage2p<9, false, true>(c, a);
+      if (!a.dtc && !a.Vout) return launch_stage2p<NV, IR, STAGE_PLAIN>(c, a);
+      if constexpr (NV == 9 && !IR) {
+        if (a.dtc) return launch_stage2p<9, false, STAGE_REPLAY>(c, a);
+      }
     }
   }
-  if (a.dtc) {  // frozen-dt replay kernels: the adjoint configuration only
+  if (a.dtc || a.Vout) {  // frozen-dt replay and saving kernels: the adjoint configuration only
     if constexpr (NDIM == 2 && NV == 9 && !IR) {
-      if (!ext) return launch_stage_tile<2, 9, false, false, true>(c, a);
+      if (!ext && a.dtc) return launch_stage_tile<2, 9, false, false, STAGE_REPLAY>(c, a);
+      if (!ext) return launch_stage_tile<2, 9, false, false, STAGE_SAVE>(c, a);
     }
-    return fail(PIP_ERR_UNSUPPORTED, "replay kernels cover 2-D, flag_divb = 1, flag_IR = 0, explicit collisions");
+    return fail(PIP_ERR_UNSUPPORTED, "replay and saving kernels cover 2-D, flag_divb = 1, flag_IR = 0, explicit collisions");
   }
   return ext ? launch_stage_tile<NDIM, NV, IR, true>(c, a) : launch_stage_tile<NDIM, NV, IR, false>(c, a);
 }
@@ -933,6 +936,39 @@ int pip_record_steps(pip_ctx* c, int n, double* hist) {
   for (int k = 0; k < n; k++) {
     hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
     TRY(run_stages(c, 2, nullptr));
+    TRY(launch_finalize(c, 1, 0));
+  }
+  HIPCHK(hipGetLastError());
+  return PIP_OK;
+}
+
+int pip_record_steps_saved(pip_ctx* c, int n, double* hist, double* Un, double* U1, double* Vn, double* V1) {
+  if (!c) return fail(PIP_ERR_ARG, "null context");
+  if (n < 0 || (n > 0 && !hist)) return fail(PIP_ERR_ARG, "n < 0 or null history");
+  if (n > 0 && !(Un && U1 && Vn && V1)) return fail(PIP_ERR_ARG, "null save buffer");
+  if (!(c->cfg.ndim == 2 && c->nv == 9 && c->cfg.flag_ir == 0 && c->cfg.t_order == 2 && col_type(c->ph) == 0 &&
+        c->cfg.flag_pip_imp == 0))
+    return fail(PIP_ERR_UNSUPPORTED, "saving steps cover 2-D, flag_divb = 1, flag_IR = 0, flag_col = 1, "
+                                     "explicit collisions, SSP-RK2");
+  if (c->arith != PIP_ARITH_EXACT) return fail(PIP_ERR_UNSUPPORTED, "saving steps run the exact arithmetic");
+  if (c->top_pending) TRY(loop_top(c, 1.0e300, (long long)1 << 62));
+  if (n == 0) return PIP_OK;
+  // the steps of pip_record_steps with the stage outputs written straight
+  // into the save buffers (as pip_replay_steps does) and the primitives of
+  // each stage input saved by the stage kernels
+  const size_t S = (size_t)c->N * c->nvt;
+  HIPCHK(hipMemcpyAsync(Un, c->U, sizeof(double) * S, hipMemcpyDeviceToDevice, c->stream));
+  for (int k = 0; k < n; k++) {
+    double* un = Un + (size_t)k * S;
+    double* u1 = U1 + (size_t)k * S;
+    double* out = (k + 1 < n) ? Un + (size_t)(k + 1) * S : c->U;
+    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
+    StageArgs a1 = stage_args(c, 1, un, un, u1, 0, nullptr);
+    StageArgs a2 = stage_args(c, 2, u1, un, out, 2, nullptr);
+    a1.Vout = Vn + (size_t)k * S;
+    a2.Vout = V1 + (size_t)k * S;
+    TRY(launch_stage(c, a1));
+    TRY(launch_stage(c, a2));
     TRY(launch_finalize(c, 1, 0));
   }
   HIPCHK(hipGetLastError());

@@ -252,6 +252,63 @@ def test_u2v_vjp_matches_finite_differences(floors):
                 assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
 
 
+def test_generated_vjps_are_the_generators_output(tmp_path):
+    """pip_adj_gen.h is what gen_adjoint.py writes (the kernels and these
+    checks compile the committed file)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_adjoint", os.path.join(build.CSRC, "gen_adjoint.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    out = tmp_path / "pip_adj_gen.h"
+    gen.main(str(out))
+    assert out.read_text() == open(os.path.join(build.CSRC, "pip_adj_gen.h")).read()
+
+
+@pytest.mark.parametrize("sm_target", [3.0e-11, 1.0e-9, 1.0e-4])
+def test_hllc_switch_derivative_at_small_contact_speeds(sm_target):
+    """The smoothed switch thm = (|S_M| + S_M) / max(2 S_M, 1e-10)
+    (HLL_rot.f90:679-681) has derivative 2e10 inside its band and zero
+    above it.  States whose contact speed S_M lies in the band or just above
+    it: the VJP equals central differences with steps small enough to stay
+    on one side of the band edge (to 1e-4: the steps are 1e-11 there)."""
+    vl = np.array([1.0, 0.0, 0.2, -0.1, 1.0])
+    vr = np.array([0.8, 0.0, -0.1, 0.3, 0.8])
+
+    def sm_of(vr1):
+        # HLLC contact speed of the pair (HLL_rot.f90:656-660 in the
+        # primitive form of the generator), the wave speeds from the
+        # sound speeds with the factor 1.03
+        w = vr.copy()
+        w[1] = vr1
+        cl, cr = np.sqrt(GM * vl[4] / vl[0]), np.sqrt(GM * w[4] / w[0])
+        sl = 1.03 * min(vl[1] - cl, w[1] - cr)
+        sr = 1.03 * max(vl[1] + cl, w[1] + cr)
+        num = w[0] * w[1] * (sr - w[1]) - vl[0] * vl[1] * (sl - vl[1]) + vl[4] - w[4]
+        den = w[0] * (sr - w[1]) - vl[0] * (sl - vl[1])
+        return num / den
+    lo, hi = -1.0, 1.0  # bisect the right state's velocity for S_M = sm_target
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if sm_of(mid) < sm_target else (lo, mid)
+    vr[1] = 0.5 * (lo + hi)
+    assert abs(sm_of(vr[1]) - sm_target) <= 1e-3 * sm_target
+    lam = RNG.normal(size=5)
+    lvl, lvr = np.zeros(5), np.zeros(5)
+    L.hc_hllc_vjp(p(vl), p(vr), GM, p(lam), p(lvl), p(lvr))
+    # steps that keep S_M on its side of the band edge (5e-11); the
+    # differences then carry round-off of ~1e-16 / h
+    h = 1e-11 if sm_target < 1e-8 else 1e-8
+    for side, v, g in (("l", vl, lvl), ("r", vr, lvr)):
+        for k in range(5):
+            vp, vm = v.copy(), v.copy()
+            vp[k] += h
+            vm[k] -= h
+            fp = hllc(vp, vr) if side == "l" else hllc(vl, vp)
+            fm = hllc(vm, vr) if side == "l" else hllc(vl, vm)
+            fd = float(lam @ (fp - fm)) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-4 * max(1.0, abs(fd)), (side, k, fd, g[k])
+
+
 def test_face_chain_vjp_matches_finite_differences():
     """The per-face chain of the adjoint face kernel: primitives of the four
     cells around a face -> minmod face states -> density floor -> HLLD (GLM)

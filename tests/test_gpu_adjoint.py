@@ -18,6 +18,7 @@ the relative L2 error ||g_fd - g_adj|| / ||g_fd|| and the median per-cell
 relative error (a cell whose gradient is near zero has a per-cell ratio
 dominated by the round-off of the two replays, not by the adjoint).
 """
+import ctypes
 import math
 
 import numpy as np
@@ -143,6 +144,102 @@ def test_saved_primitives_match_recomputed():
         grads.append(run.backward(g, ckpts, ac, hist))
         run.close()
     assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+
+
+def test_recorded_last_segment_equals_its_replay():
+    """The recording run keeps its last segment's stage inputs and their
+    primitives (pip_record_steps_saved): they equal what the replay of that
+    segment saves, the run ends on the state of a plain recording run, and
+    the gradient that skips the last replay equals the one that replays it,
+    bit for bit."""
+    from pip_amd._lib import lib
+    from pip_amd.adjoint import _ptr
+    print("case recorded last segment vs replay n=48 steps=12 ckpt=5", flush=True)
+    cfg, run, base, v0, ac, _ = _setup(48, 12, 5)
+    U0 = build_U0(cfg, base, v0)
+    out, ckpts, hist = run.forward_record(U0, ac)
+    tag = run.last_run
+    assert tag is not None
+    kept = [b[:2].clone() for b in (run._seg_Un, run._seg_U1, run._seg_Vn, run._seg_V1)]
+    run.save_primitives = False  # plain pip_record_steps for every segment
+    out_plain, _, hist_plain = run.forward_record(U0, ac)
+    assert run.last_run is None
+    assert torch.equal(out, out_plain) and torch.equal(hist[:24], hist_plain[:24])
+    run.save_primitives = True
+    bufs = [run._empty_state(2) for _ in range(4)]
+    assert lib().pip_set_state_device(run.h, _ptr(ckpts[-1])) == 0
+    assert lib().pip_replay_steps(run.h, 2, _ptr(hist) + 16 * 10, *[_ptr(b) for b in bufs]) == 0
+    run.solver.sync()
+    for a, b in zip(kept, bufs):
+        assert torch.equal(a, b)
+    g = torch.sin(out * 3.0)
+    out2, ckpts2, hist2 = run.forward_record(U0, ac)
+    grads_kept = run.backward(g, ckpts2, ac, hist2, run=run.last_run)
+    grads_replayed = run.backward(g, ckpts2, ac, hist2)
+    assert torch.equal(grads_kept[0], grads_replayed[0]) and torch.equal(grads_kept[1], grads_replayed[1])
+    run.close()
+
+
+@pytest.mark.parametrize("bnd", [(1, 1, 1, 1), (2, 2, 1, 1), (1, 1, 3, 2)])
+def test_adjoint_step_equals_host_work_items(bnd):
+    """One adjoint step through the C-ABI (pip_adj_seed, pip_adj_step) equals
+    the host build of the work items run one after another (hc_adj_step, the
+    step the CPU tests check against central differences of the oracle), on
+    a 37 x 29 grid that leaves partial gather tiles on both edges; periodic
+    and mirror boundaries.  The kernels keep the neutrals' face adjoints of a
+    tile in shared memory and the host writes them to planes; the device
+    VJPs divide through a refined reciprocal and contract the multiply-adds
+    of their reverse sweep (the host divides exactly and rounds every
+    operation), so the bar is 1e-13 of the largest entry.  It holds at the
+    mirror walls, where S_M = 0 exactly by symmetry, because the VJPs'
+    restated forward is not contracted, and at small contact speeds, where
+    the derivative of HLLC's smoothed switch is zero, because it is not
+    formed as a difference of quotients."""
+    from oracle import oracle as orc
+    from pip_amd._lib import check, lib
+    from pip_amd.config import PIPConfig
+    from tests.test_adjoint_cpu import L as HOST, _smooth_state
+    print(f"case adjoint step vs host work items 37x29 flag_bnd={bnd}", flush=True)
+    cfg = PIPConfig(ndim=2, nx=37, ny=29, xmin=0.0, xmax=1.0, ymin=0.0, ymax=29 / 37, flag_divb=1,
+                    ac_field=True, flag_bnd=bnd, col=30.0)
+    uh, um, ac = _smooth_state(cfg, 5)
+    o = orc.Oracle(cfg, uh, um, ac, prologue=False)
+    o.pipbnd()
+    uh, um = o.state()
+    dt = 1.0e-3
+    cmax = cfg.safety * min(cfg.dx, cfg.dy) / dt
+    o.set_scalars(dt, cmax)
+    U1 = np.ascontiguousarray(o.run_solver_stage1())
+    Un = np.ascontiguousarray(np.concatenate([um, uh]))
+    rng = np.random.default_rng(17)
+    W = rng.normal(size=Un.shape)  # ghost entries too: the seed folds them
+    lam_h, lam_ac_h = W.copy(), np.zeros((cfg.jx, cfg.ix))
+    P = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.c_int
+    HOST.hc_adj_step.argtypes = [I, I, P, ctypes.POINTER(I), P, P, P, P, P]
+    prm = np.array([cfg.gm, cfg.ro_lim, cfg.pr_lim, cfg.dx, cfg.dy, cfg.col, cfg.safety, cfg.db_clean, dt, cmax])
+    HOST.hc_adj_step(cfg.ix, cfg.jx, prm.ctypes.data_as(P), (I * 4)(*bnd), Un.ctypes.data_as(P),
+                     U1.ctypes.data_as(P), np.ascontiguousarray(ac).ctypes.data_as(P), lam_h.ctypes.data_as(P),
+                     lam_ac_h.ctypes.data_as(P))
+    s = PIPSolver(cfg, uh, um, ac, prologue=False)
+    dev = torch.device("cuda:0")
+    Un_d, U1_d = torch.tensor(Un, device=dev), torch.tensor(U1, device=dev)
+    dtc = torch.tensor([dt, cmax], dtype=torch.float64, device=dev)
+    lam = torch.tensor(W, device=dev)
+    lam_ac = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    check(lib().pip_adj_seed(s.h, lam.data_ptr()))
+    check(lib().pip_adj_step(s.h, Un_d.data_ptr(), U1_d.data_ptr(), None, None, dtc.data_ptr(), lam.data_ptr(),
+                             lam_ac.data_ptr()))
+    s.sync()
+    g, g_ac = lam.cpu().numpy(), lam_ac.cpu().numpy()
+    s.close()
+    scale, scale_ac = np.abs(lam_h).max(), np.abs(lam_ac_h).max()
+    err, err_ac = np.abs(g - lam_h).max() / scale, np.abs(g_ac - lam_ac_h).max() / scale_ac
+    print(f"  max |gpu - host| / max |host|: state {err:.2e}, alpha {err_ac:.2e}", flush=True)
+    assert scale > 0 and scale_ac > 0
+    assert err <= 1e-13 and err_ac <= 1e-13
+    assert np.all(g[:, :2] == 0) and np.all(g[:, -2:] == 0) and np.all(g[:, :, :2] == 0) and np.all(g[:, :, -2:] == 0)
 
 
 def test_two_pass_stage_kernels_give_the_same_gradient():
