@@ -73,9 +73,16 @@ class Block:
         self.g = g
         self.items = []  # ('let', sym, expr, kind) | ('if', [(cond, Block), ...])
 
-    def let(self, name, expr, kind="expr"):
+    def let(self, name, expr, kind="expr", exact=False):
+        """exact: the statement and the temporaries it lowers to are evaluated
+        without contraction (a sum of products that cancels bit for bit in
+        the rounded forward arithmetic stays exactly zero)."""
+        self.g._exact_mode = exact
         e = self.g.lower(self, sp.sympify(expr)) if kind == "expr" else sp.sympify(expr)
+        self.g._exact_mode = False
         s = self.g.sym(name)
+        if exact:
+            self.g.exact.add(s)
         self.items.append(("let", s, e, kind))
         return s
 
@@ -127,6 +134,8 @@ class Gen:
         self.name = name
         self.syms = {}
         self.inputs = []
+        self.exact = set()  # statements evaluated without contraction
+        self._exact_mode = False
         self.root = Block(self)
 
     def sym(self, name):
@@ -149,6 +158,8 @@ class Gen:
     def _temp(self, blk, expr, kind):
         self._ntemp = getattr(self, "_ntemp", 0) + 1
         t = self.sym(f"{self.name}{self._ntemp}")
+        if self._exact_mode:
+            self.exact.add(t)
         blk.items.append(("let", t, expr, kind))
         return t
 
@@ -203,8 +214,10 @@ class Gen:
                     out.append(f"{pad}{s} = PIP_VDIV({cc(e[0])}, {cc(e[1])});")
                 elif kind == "sqrt":  # constants fold through sqrt
                     out.append(f"{pad}{s} = {'sqrt' if e.is_Number else 'PIP_VSQRT'}({cc(e)});")
-                else:
+                elif s in self.exact:
                     out.append(f"{pad}{{ PIP_FWD_EXACT {s} = {cc(e)}; }}")
+                else:
+                    out.append(f"{pad}{s} = {cc(e)};")
             else:
                 pass  # branches handled by _emit
 
@@ -350,7 +363,9 @@ def gen_hllc():
     b2 = B.let("srb", vnr + csr)
     mx = B.maxv("srm", b1, b2)
     sr = B.let("sr", acc * mx)
-    sm = B.let("sm", (ur[1] * (sr - vnr) - ul[1] * (sl - vnl) + pl - pr) / (ur[0] * (sr - vnr) - ul[0] * (sl - vnl)))
+    # exact: mirror-image states (a mirror wall) give S_M = 0 bit for bit
+    sm = B.let("sm", (ur[1] * (sr - vnr) - ul[1] * (sl - vnl) + pl - pr) / (ur[0] * (sr - vnr) - ul[0] * (sl - vnl)),
+               exact=True)
     prm = B.let("prm", sm * ul[0] * (sl - vnl) - ul[1] * (sl - vnl) + pl)
     il = B.let("il", 1 / (sl - sm))
     ir = B.let("ir", 1 / (sr - sm))
@@ -430,7 +445,8 @@ def gen_hlld():
     glm(bA, list(fl))
     glm(bB, list(fr))
     inv = bC.let("inv", 1 / (sr - sl))
-    uh = [bC.let(f"uh{k}", inv * (sr * ur[k] - sl * ul[k] - fr[k] + fl[k])) for k in range(8)]
+    # exact: mirror-image states give uh1 = 0, so S_M = 0, bit for bit
+    uh = [bC.let(f"uh{k}", inv * (sr * ur[k] - sl * ul[k] - fr[k] + fl[k]), exact=True) for k in range(8)]
     # shared reciprocals: the star states divide by (s - sm) and by uh0
     iu0 = bC.let("iu0", 1 / uh[0])
     sm = bC.let("sm", uh[1] * iu0)
