@@ -237,6 +237,46 @@ def write_sq_summary(d: str, wl: str, n: int, tile: str, out: str, arith: str = 
     print("\n".join(lines))
 
 
+ADJ_SQ_PASSES = (("sq1", "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR"),
+                 ("sq2", "SQ_WAVE_CYCLES SQ_ACTIVE_INST_VALU SQ_WAIT_INST_ANY SQ_BUSY_CYCLES"),
+                 ("sq3", "SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 SQ_INSTS_VALU_TRANS_F64"))
+ADJ_KERNELS = (("plasma face VJPs", "k_adj_plasma_faces"), ("gather", "k_adj_gather"),
+               ("replay stage", ", 1>(pip::StageArgs)"))
+
+
+def write_adj_sq_summary(d: str, out: str):
+    """Instruction and wave-state counters of the adjoint kernels per launch
+    (the plasma face launch, the gather, a replay stage) from the rocprofv3
+    --pmc passes d/sq1..sq3 of `bench.py --only adjoint`."""
+    cmds = [f"cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc {c} --output-format csv "
+            f"-d {d}/{sub} -o run -- python3 /root/repo/bench.py --only adjoint" for sub, c in ADJ_SQ_PASSES]
+    vals = {}
+    for sub, cs in ADJ_SQ_PASSES:
+        for c in cs.split():
+            rows = _pmc_rows(os.path.join(d, sub), c)
+            vals[c] = [sum(v for k, v in rows if key in k) / max(1, sum(1 for k, _ in rows if key in k))
+                       for _, key in ADJ_KERNELS]
+    lines = ["# Adjoint kernels, Alfven-wave damping 512^2 FP64: instruction and wave-state counters", "",
+             "Commands (each its own pass, from the repository root):", ""]
+    lines += [f"    {c}" for c in cmds]
+    lines += ["", "Per launch (mean over the profiled launches; SQ_*_CYCLES count 4-cycle units, summed "
+              "over the waves):", "", "| counter | " + " | ".join(n for n, _ in ADJ_KERNELS) + " |",
+              "|---|" + "---|" * len(ADJ_KERNELS)]
+    for c, v in vals.items():
+        lines.append(f"| {c} | " + " | ".join(f"{x:.4e}" for x in v) + " |")
+    lines += ["", "Derived:", ""]
+    for i, (name, _) in enumerate(ADJ_KERNELS):
+        w, valu = vals["SQ_WAVES"][i], vals["SQ_INSTS_VALU"][i]
+        f64 = sum(vals[c][i] for c in ADJ_SQ_PASSES[2][1].split())
+        wc = vals["SQ_WAVE_CYCLES"][i]
+        lines.append(f"- {name}: {valu / w:.0f} VALU instructions per wave, {f64 / valu:.2f} of them FP64; "
+                     f"a wave issues VALU in {vals['SQ_ACTIVE_INST_VALU'][i] / wc:.2f} of its cycles and waits "
+                     f"on memory or LDS (s_waitcnt) in {vals['SQ_WAIT_INST_ANY'][i] / wc:.2f}")
+    lines.append("")
+    open(out, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
 def write_stats_summary(d: str, cmd: str, title: str, out: str, note: str = ""):
     """Kernel-trace summary (rocprofv3 --kernel-trace --stats, csv) as a
     markdown table."""
@@ -520,6 +560,9 @@ def main():
                     help="write a kernel-trace summary of the rocprofv3 --stats csv under DIR to OUT and exit")
     ap.add_argument("--sq-summary", nargs=2, metavar=("DIR", "OUT"),
                     help="write the SQ counter summary of the passes under DIR (sq1, sq2) to OUT and exit")
+    ap.add_argument("--adj-sq-summary", nargs=2, metavar=("DIR", "OUT"),
+                    help="write the adjoint kernels' SQ counter summary of the passes under DIR "
+                         "(sq1..sq3 of --only adjoint) to OUT and exit")
     ap.add_argument("--stats-cmd", default="", help="command recorded in --stats-summary")
     ap.add_argument("--stats-note", default="", help="note recorded in --stats-summary")
     ap.add_argument("--only", choices=["batched", "batched_2d", "adjoint", "copy", "ensemble"],
@@ -533,6 +576,9 @@ def main():
     if args.stats_summary:
         d, out, title = args.stats_summary
         write_stats_summary(d, args.stats_cmd, title, out, args.stats_note)
+        return
+    if args.adj_sq_summary:
+        write_adj_sq_summary(*args.adj_sq_summary)
         return
     if args.sq_summary:
         write_sq_summary(args.sq_summary[0], args.workload, n, args.tile, args.sq_summary[1],
