@@ -532,10 +532,33 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     t_rec = float(np.median([t[0] for t in times[1:]]))
     t_bwd = float(np.median([t[1] for t in times[1:]]))
     run.close()
+
+    def gradient_s(keep):
+        """median recording forward + backward with `keep` kept segments"""
+        r = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile, keep_segments=keep)
+        ts = []
+        for _ in range(4):
+            U = U0.clone().requires_grad_(True)
+            a = acT.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            UT = r.run(U, a)
+            UT.backward(UT.detach() * mask)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        r.close()
+        return float(np.median(ts[1:]))
+    seg_gb = 4 * ckpt * U0.numel() * 8 / 1e9
+    trade = {f"keep_{k}": {"gradient_over_forward": gradient_s(k) / t_fwd, "kept_GB": k * seg_gb} for k in (0, 5)}
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
             "tile": list(tile), "timing": "medians of 3 runs each",
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
-            "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd}
+            "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd,
+            "kept_segments": 1,
+            "memory_for_recompute": {"note": "the same gradient with 0 or 5 of the 10 newest segments' stage "
+                                             "inputs kept from the recording run instead of 1 (the rest replayed "
+                                             "from their checkpoints); not the configuration above",
+                                     **trade}}
 
 
 def main():

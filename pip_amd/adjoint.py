@@ -42,13 +42,18 @@ def _ptr(t: torch.Tensor) -> int:
 
 class TwoFluidRun:
     def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
-                 device: str = "cuda:0", save_primitives: bool = True, replay_tile="auto"):
+                 device: str = "cuda:0", save_primitives: bool = True, replay_tile="auto",
+                 keep_segments: int = 1):
+        """keep_segments: how many of the newest segments the recording run
+        keeps the stage inputs of (4 states per step each, 2.4 GB per
+        20-step segment at 512^2), so that the backward sweep replays only
+        the others; 0 replays every segment."""
         if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
             raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
         if not cfg.ac_field:
             raise ValueError("the adjoint run needs a per-cell ac field (cfg.ac_field)")
-        if nsteps < 0 or ckpt_every < 1:
-            raise ValueError("nsteps >= 0 and ckpt_every >= 1")
+        if nsteps < 0 or ckpt_every < 1 or keep_segments < 0:
+            raise ValueError("nsteps >= 0, ckpt_every >= 1 and keep_segments >= 0")
         self.cfg = cfg
         self.nsteps = int(nsteps)
         self.ckpt_every = int(ckpt_every)
@@ -67,11 +72,12 @@ class TwoFluidRun:
         self.replay_tile = tuple(replay_tile) if replay_tile is not None else None
         self.h = self.solver.h
         self.hist = torch.zeros(2 * max(self.nsteps, 1), dtype=torch.float64, device=self.device)
-        self._seg_Un = None
-        self._seg_U1 = None
-        self._seg_Vn = None
-        self._seg_V1 = None
-        # the forward run whose last segment the buffers hold (None: none)
+        # segment buffers: stage inputs and their primitives, 4 states per
+        # step; buffer set i holds the i-th newest kept segment, and set 0
+        # also takes the replays of the others
+        self.keep_segments = int(keep_segments)
+        self._bufs = []
+        # the forward run whose kept segments the buffers hold (None: none)
         self._seg_run = None
         self.last_run = None
         # the replay also saves the stage inputs' primitives (twice the segment
@@ -119,16 +125,21 @@ class TwoFluidRun:
         if U0.device != self.device or ac.device != self.device:
             raise ValueError(f"inputs must live on {self.device}")
 
-    def _segment_buffers(self):
-        """Stage inputs (and their primitives) of one segment, 4 states per
-        step, allocated on first use."""
+    def _kept(self):
+        """Number of newest segments the recording run keeps."""
+        return min(self.keep_segments, len(self._segments())) if self.save_primitives else 0
+
+    def _segment_buffers(self, nsets: int):
+        """The first nsets buffer sets (Un, U1, Vn, V1: m states each; Vn, V1
+        None without primitive saves), allocated on first use."""
         mmax = max((m for _, m in self._segments()), default=0)
-        if mmax and (self._seg_Un is None or self._seg_Un.shape[0] < mmax):
-            self._seg_Un = self._empty_state(mmax)
-            self._seg_U1 = self._empty_state(mmax)
-            if self.save_primitives:
-                self._seg_Vn = self._empty_state(mmax)
-                self._seg_V1 = self._empty_state(mmax)
+        if mmax and self._bufs and self._bufs[0][0].shape[0] < mmax:
+            self._bufs = []
+        while mmax and len(self._bufs) < nsets:
+            sv = self.save_primitives
+            self._bufs.append((self._empty_state(mmax), self._empty_state(mmax),
+                               self._empty_state(mmax) if sv else None, self._empty_state(mmax) if sv else None))
+        return self._bufs
 
     def _empty_state(self, n=None):
         c = self.cfg
@@ -139,26 +150,30 @@ class TwoFluidRun:
     def forward_record(self, U0: torch.Tensor, ac: torch.Tensor):
         """Forward run recording dt/cmax; returns (U_T, checkpoints, history),
         history being this run's own copy of the recorded (dt, cmax) pairs
-        (the runner's buffer is overwritten by the next forward).  The last
-        segment runs through pip_record_steps_saved, which leaves its stage
-        inputs in the segment buffers; ``last_run`` names this run for
-        backward() until another forward or a replay overwrites them."""
+        (the runner's buffer is overwritten by the next forward).  The kept
+        newest segments run through pip_record_steps_saved, which leaves
+        their stage inputs in the segment buffers; ``last_run`` names this
+        run for backward() until another forward or a replay overwrites
+        them."""
         self._start(U0, ac)
         L = lib()
         ckpts = []
         segs = self._segments()
+        kept = self._kept()
+        bufs = self._segment_buffers(kept)
         self._seg_run = None
         for s_, (k0, m) in enumerate(segs):
             ck = self._empty_state()
             check(L.pip_get_state_device(self.h, _ptr(ck)))
             ckpts.append(ck)
-            if s_ == len(segs) - 1 and self.save_primitives:
-                self._segment_buffers()
-                check(L.pip_record_steps_saved(self.h, m, _ptr(self.hist) + 16 * k0, _ptr(self._seg_Un),
-                                               _ptr(self._seg_U1), _ptr(self._seg_Vn), _ptr(self._seg_V1)))
-                self._seg_run = object()
+            i = len(segs) - 1 - s_  # i-th newest segment
+            if i < kept:
+                check(L.pip_record_steps_saved(self.h, m, _ptr(self.hist) + 16 * k0,
+                                               *[_ptr(b) for b in bufs[i]]))
             else:
                 check(L.pip_record_steps(self.h, m, _ptr(self.hist) + 16 * k0))
+        if kept:
+            self._seg_run = object()
         self.last_run = self._seg_run
         out = self._empty_state()
         check(L.pip_get_state_device(self.h, _ptr(out)))
@@ -183,33 +198,35 @@ class TwoFluidRun:
     def backward(self, gU: torch.Tensor, ckpts, ac: torch.Tensor, hist: torch.Tensor, run=None):
         """dJ/dU0, dJ/dac from dJ/dU_T by checkpointed reverse sweeps over the
         forward run that produced ckpts and hist (``run``: its ``last_run``;
-        when the segment buffers still hold that run's last segment, its
-        replay is skipped)."""
+        when the segment buffers still hold that run's kept segments, their
+        replays are skipped)."""
         if hist.numel() < 2 * self.nsteps or len(ckpts) != len(self._segments()):
             raise ValueError("checkpoints and history do not belong to a run of this runner")
         L = lib()
         lam = gU.detach().to(torch.float64).contiguous().clone()
         lam_ac = torch.zeros_like(ac)
         segs = self._segments()
-        self._segment_buffers()
-        kept = run is not None and run is self._seg_run
+        kept = self._kept() if (run is not None and run is self._seg_run) else 0
+        bufs = self._segment_buffers(max(kept, 1))
         self._torch_done()
         check(L.pip_set_ac_device(self.h, _ptr(ac)))
         check(L.pip_adj_seed(self.h, _ptr(lam)))
         sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
-        sv = self.save_primitives
         self._use_tile(self.replay_tile)
-        for s_, ((k0, m), ck) in enumerate(zip(reversed(segs), reversed(ckpts))):
-            if not (s_ == 0 and kept):
+        for i, ((k0, m), ck) in enumerate(zip(reversed(segs), reversed(ckpts))):
+            if i < kept:
+                Un, U1, Vn, V1 = bufs[i]
+            else:
+                Un, U1, Vn, V1 = bufs[0]
                 self._seg_run = None
                 check(L.pip_set_state_device(self.h, _ptr(ck)))
-                check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(self._seg_Un),
-                                         _ptr(self._seg_U1), _ptr(self._seg_Vn) if sv else None,
-                                         _ptr(self._seg_V1) if sv else None))
+                check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(Un), _ptr(U1),
+                                         _ptr(Vn) if Vn is not None else None,
+                                         _ptr(V1) if V1 is not None else None))
             for k in reversed(range(m)):
-                check(L.pip_adj_step(self.h, _ptr(self._seg_Un) + sb * k, _ptr(self._seg_U1) + sb * k,
-                                     _ptr(self._seg_Vn) + sb * k if sv else None,
-                                     _ptr(self._seg_V1) + sb * k if sv else None,
+                check(L.pip_adj_step(self.h, _ptr(Un) + sb * k, _ptr(U1) + sb * k,
+                                     _ptr(Vn) + sb * k if Vn is not None else None,
+                                     _ptr(V1) + sb * k if V1 is not None else None,
                                      _ptr(hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
         self._use_tile(self.tile)
         self._lib_done()

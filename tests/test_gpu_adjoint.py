@@ -160,7 +160,7 @@ def test_recorded_last_segment_equals_its_replay():
     out, ckpts, hist = run.forward_record(U0, ac)
     tag = run.last_run
     assert tag is not None
-    kept = [b[:2].clone() for b in (run._seg_Un, run._seg_U1, run._seg_Vn, run._seg_V1)]
+    kept = [b[:2].clone() for b in run._bufs[0]]
     run.save_primitives = False  # plain pip_record_steps for every segment
     out_plain, _, hist_plain = run.forward_record(U0, ac)
     assert run.last_run is None
@@ -178,6 +178,38 @@ def test_recorded_last_segment_equals_its_replay():
     grads_replayed = run.backward(g, ckpts2, ac, hist2)
     assert torch.equal(grads_kept[0], grads_replayed[0]) and torch.equal(grads_kept[1], grads_replayed[1])
     run.close()
+
+
+def test_kept_segments_give_the_same_gradient():
+    """Keeping 0, 1 or 3 of the newest segments from the recording run (the
+    rest replayed) gives the same gradient bit for bit; an autograd forward
+    on the runner between a forward and its backward makes that backward
+    replay every segment."""
+    print("case adjoint keep_segments 0 / 1 / 3 n=48 steps=12 ckpt=5 (3 segments)", flush=True)
+    from pip_amd.adjoint import TwoFluidRun
+    outs = []
+    for keep in (0, 1, 3):
+        cfg, run0, base, v0, ac, _ = _setup(48, 12, 5)
+        run0.close()
+        run = TwoFluidRun(cfg, 12, ckpt_every=5, keep_segments=keep)
+        U0 = build_U0(cfg, base, v0).requires_grad_(True)
+        a = ac.clone().requires_grad_(True)
+        UT = run.run(U0, a)
+        assert (run.last_run is None) == (keep == 0)
+        UT.backward(torch.cos(UT.detach()))
+        outs.append((UT.detach(), U0.grad, a.grad))
+        if keep == 3:  # a second forward overwrites the kept segments of the first
+            U1 = build_U0(cfg, base, v0).requires_grad_(True)
+            a1 = ac.clone().requires_grad_(True)
+            UTa = run.run(U1, a1)
+            UTb = run.run(build_U0(cfg, base, 1.1 * v0), ac)
+            UTa.backward(torch.cos(UTa.detach()))
+            assert torch.equal(U1.grad, U0.grad) and torch.equal(a1.grad, a.grad)
+            del UTb
+        run.close()
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert torch.equal(x, y)
 
 
 @pytest.mark.parametrize("bnd", [(1, 1, 1, 1), (2, 2, 1, 1), (1, 1, 3, 2)])
