@@ -533,9 +533,9 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     t_bwd = float(np.median([t[1] for t in times[1:]]))
     run.close()
 
-    def gradient_s(keep):
+    def gradient_s(keep, overlap=True):
         """median recording forward + backward with `keep` kept segments"""
-        r = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile, keep_segments=keep)
+        r = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile, keep_segments=keep, overlap_replay=overlap)
         ts = []
         for _ in range(4):
             U = U0.clone().requires_grad_(True)
@@ -550,11 +550,16 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         return float(np.median(ts[1:]))
     seg_gb = 4 * ckpt * U0.numel() * 8 / 1e9
     trade = {f"keep_{k}": {"gradient_over_forward": gradient_s(k) / t_fwd, "kept_GB": k * seg_gb} for k in (0, 5)}
+    in_line = gradient_s(1, overlap=False) / t_fwd
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
             "tile": list(tile), "timing": "medians of 3 runs each",
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
             "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd,
             "kept_segments": 1,
+            "replay_overlap": {"note": "each segment's replay runs on a second context's stream beside the "
+                                       "newer segment's adjoint steps (two buffer sets); in_line: the replays "
+                                       "in the adjoint's stream, one set",
+                               "buffer_GB": 2 * seg_gb, "in_line_gradient_over_forward": in_line},
             "memory_for_recompute": {"note": "the same gradient with 0 or 5 of the 10 newest segments' stage "
                                              "inputs kept from the recording run instead of 1 (the rest replayed "
                                              "from their checkpoints); not the configuration above",

@@ -16,7 +16,10 @@ final state and is differentiable with respect to both inputs:
             saving the stage inputs (the last segment's are still there
             from the forward), then pull the adjoint back through its steps
             (hand-written reverse-mode kernels: stage update, HLLD/HLLC
-            fluxes, minmod reconstruction, collision source).
+            fluxes, minmod reconstruction, collision source).  By default
+            the replay of the next older segment runs on a second library
+            context's stream while the adjoint steps of the current one run,
+            into the other of two buffer sets (replay_sets).
 
 The gradient is that of the discrete map with the recorded time-step
 sequence held fixed (dt and the GLM speed cmax are treated as constants);
@@ -40,14 +43,44 @@ def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
+def replay_sets(nseg: int, kept: int, overlap: bool):
+    """Buffer sets of the backward sweep over nseg segments, i-th newest
+    first: (sets, waits).  Segment i reads set sets[i]; the kept newest
+    segments (i < kept) hold sets 0..kept-1 from the recording run.  In line
+    (overlap False) every replayed segment uses set 0.  Overlapped, the
+    replay of segment i + 1 runs while the adjoint steps of segment i read
+    their set, so neighbours take different sets: a replayed segment reuses
+    the set of segment i - 2, and waits[i] names the last segment that read
+    it before (the replay waits for its adjoint steps; -1: none)."""
+    if nseg < 0 or kept < 0 or kept > nseg:
+        raise ValueError("0 <= kept <= nseg")
+    sets = []
+    for i in range(nseg):
+        if i < kept:
+            sets.append(i)
+        elif not overlap:
+            sets.append(0)
+        else:
+            sets.append(i if i < 2 else sets[i - 2])
+    waits = []
+    for i in range(nseg):
+        prev = [j for j in range(i) if sets[j] == sets[i]]
+        waits.append(prev[-1] if (overlap and i >= kept and prev) else -1)
+    return sets, waits
+
+
 class TwoFluidRun:
     def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
                  device: str = "cuda:0", save_primitives: bool = True, replay_tile="auto",
-                 keep_segments: int = 1):
+                 keep_segments: int = 1, overlap_replay: bool = True):
         """keep_segments: how many of the newest segments the recording run
         keeps the stage inputs of (4 states per step each, 2.4 GB per
         20-step segment at 512^2), so that the backward sweep replays only
-        the others; 0 replays every segment."""
+        the others; 0 replays every segment.  overlap_replay: the backward
+        sweep replays each segment on a second context's stream while the
+        adjoint steps of the newer one run (bitwise the same gradient; at
+        least two buffer sets, so one more set than keep_segments = 1 needs);
+        False replays in line, into one set."""
         if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
             raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
         if not cfg.ac_field:
@@ -83,6 +116,8 @@ class TwoFluidRun:
         # the replay also saves the stage inputs' primitives (twice the segment
         # memory, no primitive pass in the adjoint step)
         self.save_primitives = bool(save_primitives)
+        self.overlap_replay = bool(overlap_replay)
+        self._rsolver = None  # the replay context (overlap_replay), made on first use
 
     # ------------------------------------------------------------ helpers
     def _segments(self):
@@ -207,30 +242,81 @@ class TwoFluidRun:
         lam_ac = torch.zeros_like(ac)
         segs = self._segments()
         kept = self._kept() if (run is not None and run is self._seg_run) else 0
-        bufs = self._segment_buffers(max(kept, 1))
+        overlap = self.overlap_replay and len(segs) > kept
+        sets, waits = replay_sets(len(segs), kept, overlap)
+        bufs = self._segment_buffers(max(sets, default=-1) + 1)
         self._torch_done()
         check(L.pip_set_ac_device(self.h, _ptr(ac)))
         check(L.pip_adj_seed(self.h, _ptr(lam)))
         sb = self.nvt * self.cfg.ix * self.cfg.jx * 8  # bytes per saved state
-        self._use_tile(self.replay_tile)
-        for i, ((k0, m), ck) in enumerate(zip(reversed(segs), reversed(ckpts))):
-            if i < kept:
-                Un, U1, Vn, V1 = bufs[i]
+        rsegs = list(zip(reversed(segs), reversed(ckpts)))  # i-th newest first
+        if overlap:
+            rs = self._replay_solver()
+            check(L.pip_set_ac_device(rs.h, _ptr(ac)))
+            rctx = rs.h
+            main_s = torch.cuda.ExternalStream(self.solver.stream(), device=self.device)
+            rep_s = torch.cuda.ExternalStream(rs.stream(), device=self.device)
+        else:
+            rctx = self.h
+            self._use_tile(self.replay_tile)
+        adj_done, rep_done = {}, {}
+
+        def replay(i):
+            # segment i from its checkpoint into buffer set sets[i]; with the
+            # overlap, after the adjoint steps of the last segment that read
+            # that set (waits[i]) and on the replay context's stream
+            (k0, m), ck = rsegs[i]
+            Un, U1, Vn, V1 = bufs[sets[i]]
+            self._seg_run = None  # a replay overwrites the kept buffers
+            if overlap and waits[i] >= 0:
+                rep_s.wait_event(adj_done[waits[i]])
+            check(L.pip_set_state_device(rctx, _ptr(ck)))
+            check(L.pip_replay_steps(rctx, m, _ptr(hist) + 16 * k0, _ptr(Un), _ptr(U1),
+                                     _ptr(Vn) if Vn is not None else None,
+                                     _ptr(V1) if V1 is not None else None))
+            if overlap:
+                rep_done[i] = torch.cuda.Event()
+                rep_done[i].record(rep_s)
+
+        try:
+            if overlap and kept == 0:
+                replay(0)
+            for i, ((k0, m), _) in enumerate(rsegs):
+                if overlap:
+                    if i + 1 < len(rsegs) and i + 1 >= kept:
+                        replay(i + 1)  # runs beside this segment's adjoint steps
+                    if i >= kept:
+                        main_s.wait_event(rep_done[i])
+                elif i >= kept:
+                    replay(i)
+                Un, U1, Vn, V1 = bufs[sets[i]]
+                for k in reversed(range(m)):
+                    check(L.pip_adj_step(self.h, _ptr(Un) + sb * k, _ptr(U1) + sb * k,
+                                         _ptr(Vn) + sb * k if Vn is not None else None,
+                                         _ptr(V1) + sb * k if V1 is not None else None,
+                                         _ptr(hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
+                if overlap:
+                    adj_done[i] = torch.cuda.Event()
+                    adj_done[i].record(main_s)
+        finally:
+            if overlap:
+                self._rsolver.sync()
             else:
-                Un, U1, Vn, V1 = bufs[0]
-                self._seg_run = None
-                check(L.pip_set_state_device(self.h, _ptr(ck)))
-                check(L.pip_replay_steps(self.h, m, _ptr(hist) + 16 * k0, _ptr(Un), _ptr(U1),
-                                         _ptr(Vn) if Vn is not None else None,
-                                         _ptr(V1) if V1 is not None else None))
-            for k in reversed(range(m)):
-                check(L.pip_adj_step(self.h, _ptr(Un) + sb * k, _ptr(U1) + sb * k,
-                                     _ptr(Vn) + sb * k if Vn is not None else None,
-                                     _ptr(V1) + sb * k if V1 is not None else None,
-                                     _ptr(hist) + 16 * (k0 + k), _ptr(lam), _ptr(lam_ac)))
-        self._use_tile(self.tile)
-        self._lib_done()
+                self._use_tile(self.tile)
+            self._lib_done()
         return lam, lam_ac
+
+    def _replay_solver(self) -> PIPSolver:
+        """The second library context the overlapped replays run on (its own
+        stream and state; the replay tile)."""
+        if self._rsolver is None:
+            c = self.cfg
+            z = np.zeros((c.jx, c.ix))
+            self._rsolver = PIPSolver(c, np.zeros((5, c.jx, c.ix)), np.zeros((c.nvar_m, c.jx, c.ix)), z,
+                                      prologue=False, tile=None)
+            if self.replay_tile is not None:
+                self._rsolver.set_tile(*self.replay_tile)
+        return self._rsolver
 
     def run(self, U0: torch.Tensor, ac: torch.Tensor) -> torch.Tensor:
         """Differentiable run: U_T = run(U0, ac)."""
@@ -241,6 +327,9 @@ class TwoFluidRun:
         return self.hist[: 2 * self.nsteps].view(-1, 2)[:, 0].cpu().numpy()
 
     def close(self):
+        if self._rsolver is not None:
+            self._rsolver.close()
+            self._rsolver = None
         self.solver.close()
 
 

@@ -212,6 +212,32 @@ def test_kept_segments_give_the_same_gradient():
             assert torch.equal(x, y)
 
 
+def test_overlapped_replays_give_the_in_line_gradient():
+    """Replays on the second context's stream beside the adjoint steps
+    (overlap_replay, two or more buffer sets reused in turn) give the
+    gradient of the in-line sweep bit for bit, with 0, 1 and 3 kept
+    segments of 5 (uneven: the last one is shorter), and twice in a row on
+    one runner."""
+    print("case adjoint overlap_replay vs in line, keep 0 / 1 / 3, n=48 steps=14 ckpt=3 (5 segments)", flush=True)
+    from pip_amd.adjoint import TwoFluidRun
+    for keep in (0, 1, 3):
+        res = []
+        for overlap in (False, True):
+            cfg, run0, base, v0, ac, _ = _setup(48, 14, 3)
+            run0.close()
+            run = TwoFluidRun(cfg, 14, ckpt_every=3, keep_segments=keep, overlap_replay=overlap)
+            for _ in range(2):
+                U0 = build_U0(cfg, base, v0).requires_grad_(True)
+                a = ac.clone().requires_grad_(True)
+                UT = run.run(U0, a)
+                UT.backward(torch.cos(UT.detach()))
+                res.append((UT.detach(), U0.grad, a.grad))
+            run.close()
+        for o in res[1:]:
+            for x, y in zip(res[0], o):
+                assert torch.equal(x, y), keep
+
+
 @pytest.mark.parametrize("bnd", [(1, 1, 1, 1), (2, 2, 1, 1), (1, 1, 3, 2)])
 def test_adjoint_step_equals_host_work_items(bnd):
     """One adjoint step through the C-ABI (pip_adj_seed, pip_adj_step) equals

@@ -56,3 +56,33 @@ def test_order_one_classification():
     assert 6 not in big and 8 not in big       # B_y, psi: born from the perturbation
     cfg, uh, um, _ = problems.orszag_tang(32)
     assert set(order_one(um, cfg)) >= {0, 1, 2, 4, 5, 6}
+
+
+def test_replay_sets_never_hand_a_replay_a_set_in_use():
+    """The backward sweep's buffer sets (pip_amd.adjoint.replay_sets): the
+    kept newest segments hold their own sets; overlapped, the replay of
+    segment i + 1 (running while segment i's adjoint steps read its set)
+    writes another set, and only after the adjoint steps of every older
+    reader of its set (it waits for the last of them, which is at least two
+    segments back); in line, the replays share set 0."""
+    import pytest
+
+    from pip_amd.adjoint import replay_sets
+    for nseg in range(0, 13):
+        for kept in range(0, nseg + 1):
+            sets, waits = replay_sets(nseg, kept, False)
+            assert sets == [i if i < kept else 0 for i in range(nseg)] and waits == [-1] * nseg
+            sets, waits = replay_sets(nseg, kept, True)
+            assert sets[:kept] == list(range(kept))
+            nsets = len(set(sets))
+            assert nsets == (max(kept, min(nseg, 2)) if nseg > kept else kept)
+            for i in range(kept, nseg):
+                if i >= 1:
+                    assert sets[i] != sets[i - 1]  # written while segment i - 1 reads its set
+                readers = [j for j in range(i) if sets[j] == sets[i]]
+                assert waits[i] == (readers[-1] if readers else -1)
+                assert waits[i] == -1 or waits[i] <= i - 2  # enqueued before this replay
+                # no kept segment still to be read (newer index = read earlier)
+                assert all(j < i for j in range(nseg) if j < kept and sets[j] == sets[i])
+    with pytest.raises(ValueError):
+        replay_sets(3, 4, True)
