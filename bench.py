@@ -277,6 +277,62 @@ def write_adj_sq_summary(d: str, out: str):
     print("\n".join(lines))
 
 
+# The adjoint leg's gradients in the order adjoint_leg runs them: (schedule,
+# gradients, replayed steps per gradient) at 200 steps, checkpoints every 20.
+ADJ_GROUPS = [("last segment kept, replays beside the adjoint steps (default)", 4, 180),
+              ("no segment kept, replays beside the adjoint steps", 4, 200),
+              ("5 segments kept, replays beside the adjoint steps", 4, 100),
+              ("last segment kept, replays in line", 4, 180)]
+
+
+def write_adj_trace_summary(d: str, out: str, nsteps: int = 200):
+    """Per-schedule mean durations of the adjoint launches and the replay
+    stages, from the kernel trace d/run_kernel_trace.csv of
+    `bench.py --only adjoint` (launches taken in start order and split by the
+    counts each schedule launches: 2 per step for each adjoint kernel, 2 per
+    replayed step)."""
+    import csv
+    import glob
+    path = sorted(glob.glob(os.path.join(d, "*kernel_trace.csv")))[0]
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    kinds = {"plasma face VJPs": lambda n: "k_adj_plasma_faces" in n,
+             "gather": lambda n: "k_adj_gather" in n,
+             "replay stage": lambda n: n.endswith(", 1>(pip::StageArgs)")}
+    seqs = {k: [(e - b) / 1e3 for b, e, n in rows if f(n)] for k, f in kinds.items()}
+    cmd = (f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --output-format csv -d {d} -o run -- "
+           "python3 /root/repo/bench.py --only adjoint")
+    lines = ["# Adjoint launches per checkpoint schedule, Alfven-wave damping 512^2, 200 steps: kernel trace",
+             "", "Command (from the repository root):", "", f"    {cmd}", "",
+             "Mean launch durations (us) of each schedule's gradients, the launches taken in start order and "
+             "split by the counts each schedule makes (2 per step for each adjoint kernel, 2 per replayed step). "
+             "With the replays beside the adjoint steps the launches of the two streams share the GPU, so "
+             "their durations overlap; the in-line schedule gives each launch's standalone duration.", "",
+             "| schedule | gradients | plasma face VJPs | gather | replay stage | replay launches |",
+             "|---|---|---|---|---|---|"]
+    pos = {k: 0 for k in seqs}
+    for name, ng, rep in ADJ_GROUPS:
+        counts = {"plasma face VJPs": 2 * nsteps * ng, "gather": 2 * nsteps * ng, "replay stage": 2 * rep * ng}
+        means = {}
+        for k, c in counts.items():
+            seg = seqs[k][pos[k]:pos[k] + c]
+            if len(seg) != c:
+                raise SystemExit(f"{path}: {k}: {len(seqs[k])} launches, fewer than the schedules make")
+            pos[k] += c
+            means[k] = sum(seg) / c
+        lines.append(f"| {name} | {ng} | {means['plasma face VJPs']:.1f} | {means['gather']:.1f} | "
+                     f"{means['replay stage']:.1f} | {counts['replay stage']} |")
+    extra = {k: len(v) - pos[k] for k, v in seqs.items()}
+    if any(extra.values()):
+        raise SystemExit(f"{path}: launches beyond the schedules: {extra}")
+    lines.append("")
+    open(out, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
 def write_stats_summary(d: str, cmd: str, title: str, out: str, note: str = ""):
     """Kernel-trace summary (rocprofv3 --kernel-trace --stats, csv) as a
     markdown table."""
@@ -588,6 +644,9 @@ def main():
                     help="write a kernel-trace summary of the rocprofv3 --stats csv under DIR to OUT and exit")
     ap.add_argument("--sq-summary", nargs=2, metavar=("DIR", "OUT"),
                     help="write the SQ counter summary of the passes under DIR (sq1, sq2) to OUT and exit")
+    ap.add_argument("--adj-trace-summary", nargs=2, metavar=("DIR", "OUT"),
+                    help="write the per-schedule launch durations of the adjoint leg's kernel trace under DIR "
+                         "(rocprofv3 --kernel-trace of --only adjoint) to OUT and exit")
     ap.add_argument("--adj-sq-summary", nargs=2, metavar=("DIR", "OUT"),
                     help="write the adjoint kernels' SQ counter summary of the passes under DIR "
                          "(sq1..sq3 of --only adjoint) to OUT and exit")
@@ -607,6 +666,9 @@ def main():
         return
     if args.adj_sq_summary:
         write_adj_sq_summary(*args.adj_sq_summary)
+        return
+    if args.adj_trace_summary:
+        write_adj_trace_summary(*args.adj_trace_summary)
         return
     if args.sq_summary:
         write_sq_summary(args.sq_summary[0], args.workload, n, args.tile, args.sq_summary[1],
