@@ -279,7 +279,7 @@ def write_adj_sq_summary(d: str, out: str):
 
 # The adjoint leg's gradients in the order adjoint_leg runs them: (schedule,
 # gradients, replayed steps per gradient) at 200 steps, checkpoints every 20.
-ADJ_GROUPS = [("last segment kept, replays beside the adjoint steps (default)", 4, 180),
+ADJ_GROUPS = [("2 newest segments kept, replays beside the adjoint steps (default)", 4, 160),
               ("no segment kept, replays beside the adjoint steps", 4, 200),
               ("5 segments kept, replays beside the adjoint steps", 4, 100),
               ("last segment kept, replays in line", 4, 180)]
@@ -587,6 +587,7 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         times.append((t1 - t0, t2 - t1))
     t_rec = float(np.median([t[0] for t in times[1:]]))
     t_bwd = float(np.median([t[1] for t in times[1:]]))
+    run_keep = run.keep_segments
     run.close()
 
     def gradient_s(keep, overlap=True):
@@ -606,19 +607,21 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         return float(np.median(ts[1:]))
     seg_gb = 4 * ckpt * U0.numel() * 8 / 1e9
     trade = {f"keep_{k}": {"gradient_over_forward": gradient_s(k) / t_fwd, "kept_GB": k * seg_gb} for k in (0, 5)}
-    in_line = gradient_s(1, overlap=False) / t_fwd
+    in_line = gradient_s(None, overlap=False) / t_fwd
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
             "tile": list(tile), "timing": "medians of 3 runs each",
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
             "gradient_over_forward": (t_rec + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd,
-            "kept_segments": 1,
-            "replay_overlap": {"note": "each segment's replay runs on a second context's stream beside the "
-                                       "newer segment's adjoint steps (two buffer sets); in_line: the replays "
-                                       "in the adjoint's stream, one set",
+            "kept_segments": run_keep,
+            "replay_overlap": {"note": "each replayed segment runs on a second context's stream beside the "
+                                       "newer segment's adjoint steps, in two buffer sets that the recording "
+                                       "run fills with the 2 newest segments; in_line: the replays in the "
+                                       "adjoint's stream, one set, the newest segment kept",
                                "buffer_GB": 2 * seg_gb, "in_line_gradient_over_forward": in_line},
             "memory_for_recompute": {"note": "the same gradient with 0 or 5 of the 10 newest segments' stage "
-                                             "inputs kept from the recording run instead of 1 (the rest replayed "
-                                             "from their checkpoints); not the configuration above",
+                                             "inputs kept from the recording run instead of 2 (the rest replayed "
+                                             "from their checkpoints; 5 sets hold more memory than the two the "
+                                             "sweep needs); not the configuration above",
                                      **trade}}
 
 

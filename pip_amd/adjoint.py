@@ -9,11 +9,13 @@ collision coefficient field ``ac`` (jx, ix).  ``run(U0, ac)`` returns the
 final state and is differentiable with respect to both inputs:
 
   forward   records dt/cmax of every step on the device and keeps the state
-            at every ``ckpt_every``-th step (step checkpointing); the last
-            segment also keeps its stage inputs (what its replay would save);
+            at every ``ckpt_every``-th step (step checkpointing); the newest
+            segments, as many as the backward sweep has buffer sets (two by
+            default), also keep their stage inputs (what their replays
+            would save);
   backward  for each segment, newest first: restore its checkpoint, replay
             the segment with the recorded dt (bitwise the forward run)
-            saving the stage inputs (the last segment's are still there
+            saving the stage inputs (the newest segments' are still there
             from the forward), then pull the adjoint back through its steps
             (hand-written reverse-mode kernels: stage update, HLLD/HLLC
             fluxes, minmod reconstruction, collision source).  By default
@@ -72,19 +74,22 @@ def replay_sets(nseg: int, kept: int, overlap: bool):
 class TwoFluidRun:
     def __init__(self, cfg: PIPConfig, nsteps: int, ckpt_every: int = 20, tile=None,
                  device: str = "cuda:0", save_primitives: bool = True, replay_tile="auto",
-                 keep_segments: int = 1, overlap_replay: bool = True):
+                 keep_segments: int | None = None, overlap_replay: bool = True):
         """keep_segments: how many of the newest segments the recording run
         keeps the stage inputs of (4 states per step each, 2.4 GB per
         20-step segment at 512^2), so that the backward sweep replays only
         the others; 0 replays every segment.  overlap_replay: the backward
         sweep replays each segment on a second context's stream while the
         adjoint steps of the newer one run (bitwise the same gradient; at
-        least two buffer sets, so one more set than keep_segments = 1 needs);
-        False replays in line, into one set."""
+        least two buffer sets); False replays in line, into one set.  The
+        default keep_segments fills the buffer sets the backward sweep holds
+        anyway: 2 with overlap_replay, 1 without."""
         if cfg.ndim != 2 or cfg.nvar_m != 9 or cfg.flag_ir != 0 or cfg.t_order != 2:
             raise ValueError("the adjoint covers 2-D, flag_divb=1, flag_IR=0, SSP-RK2")
         if not cfg.ac_field:
             raise ValueError("the adjoint run needs a per-cell ac field (cfg.ac_field)")
+        if keep_segments is None:
+            keep_segments = 2 if overlap_replay else 1
         if nsteps < 0 or ckpt_every < 1 or keep_segments < 0:
             raise ValueError("nsteps >= 0, ckpt_every >= 1 and keep_segments >= 0")
         self.cfg = cfg
