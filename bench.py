@@ -277,15 +277,21 @@ def write_adj_sq_summary(d: str, out: str):
     print("\n".join(lines))
 
 
-# The adjoint leg's gradients in the order adjoint_leg runs them: (schedule,
-# gradients, replayed steps per gradient) at 200 steps, checkpoints every 20.
-ADJ_GROUPS = [("2 newest segments kept, replays beside the adjoint steps (default)", 4, 160),
-              ("no segment kept, replays beside the adjoint steps", 4, 200),
-              ("5 segments kept, replays beside the adjoint steps", 4, 100),
-              ("last segment kept, replays in line", 4, 180)]
+# The adjoint leg's checkpoint schedules, in the order adjoint_leg times
+# them (4 gradients each): (key, label, keep_segments, overlap_replay);
+# keep_segments None is TwoFluidRun's default (2 overlapped, 1 in line).
+ADJ_SCHEDULES = [("default", "2 newest segments kept, replays beside the adjoint steps (default)", None, True),
+                 ("keep_0", "no segment kept, replays beside the adjoint steps", 0, True),
+                 ("keep_5", "5 segments kept, replays beside the adjoint steps", 5, True),
+                 ("in_line", "newest segment kept, replays in line", None, False)]
+ADJ_GRADIENTS = 4
 
 
-def write_adj_trace_summary(d: str, out: str, nsteps: int = 200):
+def _adj_kept(keep, overlap, nseg):
+    return min(nseg, (2 if overlap else 1) if keep is None else keep)
+
+
+def write_adj_trace_summary(d: str, out: str, nsteps: int = 200, ckpt: int = 20):
     """Per-schedule mean durations of the adjoint launches and the replay
     stages, from the kernel trace d/run_kernel_trace.csv of
     `bench.py --only adjoint` (launches taken in start order and split by the
@@ -314,7 +320,10 @@ def write_adj_trace_summary(d: str, out: str, nsteps: int = 200):
              "| schedule | gradients | plasma face VJPs | gather | replay stage | replay launches |",
              "|---|---|---|---|---|---|"]
     pos = {k: 0 for k in seqs}
-    for name, ng, rep in ADJ_GROUPS:
+    nseg = (nsteps + ckpt - 1) // ckpt
+    for _, name, keep, overlap in ADJ_SCHEDULES:
+        ng = ADJ_GRADIENTS
+        rep = nsteps - _adj_kept(keep, overlap, nseg) * ckpt  # replayed steps (segments of ckpt steps)
         counts = {"plasma face VJPs": 2 * nsteps * ng, "gather": 2 * nsteps * ng, "replay stage": 2 * rep * ng}
         means = {}
         for k, c in counts.items():
@@ -567,47 +576,37 @@ def adjoint_leg(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         s.close()
     t_fwd = float(np.median(fwd))
     dev = torch.device("cuda:0")
-    run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile)
     U0 = torch.tensor(np.concatenate([um, uh]), device=dev)
     acT = torch.tensor(ac, device=dev)
     mask = torch.zeros((cfg.jx, cfg.ix), dtype=torch.float64, device=dev)
     mask[2:-2, 2:-2] = 1.0
-    times = []
-    for _ in range(4):  # the first pass allocates the adjoint buffers
-        U = U0.clone().requires_grad_(True)
-        a = acT.clone().requires_grad_(True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        UT = run.run(U, a)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        UT.backward(UT.detach() * mask)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        times.append((t1 - t0, t2 - t1))
-    t_rec = float(np.median([t[0] for t in times[1:]]))
-    t_bwd = float(np.median([t[1] for t in times[1:]]))
-    run_keep = run.keep_segments
-    run.close()
 
-    def gradient_s(keep, overlap=True):
-        """median recording forward + backward with `keep` kept segments"""
+    def gradients(keep, overlap):
+        """(recording forward, backward) medians of ADJ_GRADIENTS gradients,
+        the first untimed (it allocates the adjoint buffers), and the kept
+        segment count"""
         r = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, tile=tile, keep_segments=keep, overlap_replay=overlap)
         ts = []
-        for _ in range(4):
+        for _ in range(ADJ_GRADIENTS):
             U = U0.clone().requires_grad_(True)
             a = acT.clone().requires_grad_(True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             UT = r.run(U, a)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
             UT.backward(UT.detach() * mask)
             torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t0)
+            ts.append((t1 - t0, time.perf_counter() - t1))
+        kept = r.keep_segments
         r.close()
-        return float(np.median(ts[1:]))
+        return float(np.median([t[0] for t in ts[1:]])), float(np.median([t[1] for t in ts[1:]])), kept
+    res = {key: gradients(keep, ov) for key, _, keep, ov in ADJ_SCHEDULES}
+    t_rec, t_bwd, run_keep = res["default"]
     seg_gb = 4 * ckpt * U0.numel() * 8 / 1e9
-    trade = {f"keep_{k}": {"gradient_over_forward": gradient_s(k) / t_fwd, "kept_GB": k * seg_gb} for k in (0, 5)}
-    in_line = gradient_s(None, overlap=False) / t_fwd
+    trade = {k: {"gradient_over_forward": (res[k][0] + res[k][1]) / t_fwd, "kept_GB": res[k][2] * seg_gb}
+             for k in ("keep_0", "keep_5")}
+    in_line = (res["in_line"][0] + res["in_line"][1]) / t_fwd
     return {"workload": f"Alfven-wave damping {n}x{n}, {nsteps} SSP-RK2 steps, checkpoints every {ckpt}",
             "tile": list(tile), "timing": "medians of 3 runs each",
             "forward_s": t_fwd, "forward_recorded_s": t_rec, "backward_s": t_bwd,
