@@ -305,7 +305,8 @@ class TwoFluidRun:
                     adj_done[i].record(main_s)
         finally:
             if overlap:
-                self._rsolver.sync()
+                if self._rsolver is not None:
+                    self._rsolver.sync()
             else:
                 self._use_tile(self.tile)
             self._lib_done()
