@@ -288,7 +288,8 @@ ADJ_GRADIENTS = 4
 
 
 def _adj_kept(keep, overlap, nseg):
-    return min(nseg, (2 if overlap else 1) if keep is None else keep)
+    from pip_amd.adjoint import default_keep_segments
+    return min(nseg, default_keep_segments(overlap) if keep is None else keep)
 
 
 def write_adj_trace_summary(d: str, out: str, nsteps: int = 200, ckpt: int = 20):

@@ -45,6 +45,12 @@ def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
+def default_keep_segments(overlap_replay: bool) -> int:
+    """Segments the recording run keeps by default: as many as the backward
+    sweep has buffer sets (two with overlapped replays, one in line)."""
+    return 2 if overlap_replay else 1
+
+
 def replay_sets(nseg: int, kept: int, overlap: bool):
     """Buffer sets of the backward sweep over nseg segments, i-th newest
     first: (sets, waits).  Segment i reads set sets[i]; the kept newest
@@ -89,7 +95,7 @@ class TwoFluidRun:
         if not cfg.ac_field:
             raise ValueError("the adjoint run needs a per-cell ac field (cfg.ac_field)")
         if keep_segments is None:
-            keep_segments = 2 if overlap_replay else 1
+            keep_segments = default_keep_segments(overlap_replay)
         if nsteps < 0 or ckpt_every < 1 or keep_segments < 0:
             raise ValueError("nsteps >= 0, ckpt_every >= 1 and keep_segments >= 0")
         self.cfg = cfg
