@@ -238,6 +238,32 @@ def test_overlapped_replays_give_the_in_line_gradient():
                 assert torch.equal(x, y), keep
 
 
+@pytest.mark.parametrize("nsteps,ckpt", [(0, 5), (3, 5), (5, 5), (6, 5)])
+def test_short_runs_have_the_in_line_gradient(nsteps, ckpt):
+    """Runs of no steps, of one segment shorter than or equal to the
+    checkpoint interval, and of a full segment plus one step: the default
+    schedule (overlapped replays, two kept segments) gives the in-line
+    gradient bit for bit, and with no steps alpha_c has no gradient."""
+    print(f"case adjoint short run n=48 steps={nsteps} ckpt={ckpt}", flush=True)
+    from pip_amd.adjoint import TwoFluidRun
+    res = []
+    for overlap in (True, False):
+        cfg, run0, base, v0, ac, _ = _setup(48, max(nsteps, 1), ckpt)
+        run0.close()
+        run = TwoFluidRun(cfg, nsteps, ckpt_every=ckpt, overlap_replay=overlap)
+        U0 = build_U0(cfg, base, v0).requires_grad_(True)
+        a = ac.clone().requires_grad_(True)
+        UT = run.run(U0, a)
+        g = torch.cos(UT.detach())
+        UT.backward(g)
+        res.append((UT.detach(), U0.grad, a.grad))
+        if nsteps == 0:
+            assert not bool(a.grad.abs().max() > 0)
+        run.close()
+    for x, y in zip(*res):
+        assert torch.equal(x, y)
+
+
 @pytest.mark.parametrize("bnd", [(1, 1, 1, 1), (2, 2, 1, 1), (1, 1, 3, 2)])
 def test_adjoint_step_equals_host_work_items(bnd):
     """One adjoint step through the C-ABI (pip_adj_seed, pip_adj_step) equals
