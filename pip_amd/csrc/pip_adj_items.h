@@ -78,9 +78,9 @@ PIP_HD bool adj_interior(const AdjArgs& a, int i, int j) {
 }
 
 // adjoint of the stage output at an interior cell, psi scaled by the damping
+PIP_HD double adj_mu_of(int n, double m, double psi_fac) { return (n == 8) ? m * psi_fac : m; }
 PIP_HD double adj_mu(const AdjArgs& a, int n, long long o, double psi_fac) {
-  const double m = a.lam_out[n * a.N + o];
-  return (n == 8) ? m * psi_fac : m;
+  return adj_mu_of(n, a.lam_out[n * a.N + o], psi_fac);
 }
 
 // Per cell (all cells): primitives of Uin for the face items (unless the
@@ -181,6 +181,92 @@ struct alignas(16) FacePair {
   double l, r;
 };
 
+// A face's VJP in three parts, so that a thread can issue the stencil loads
+// of a second face before the first face's VJP (k_adj_plasma_faces):
+//   adj_face_raw       the loads: the 4 primitive values of the stencil and
+//                      the stage-output adjoints of the two cells (0 at
+//                      cells that are not interior) per variable;
+//   adj_face_from_raw  the reconstructed face states, lamF = coef (mu_R -
+//                      mu_L) (psi scaled by the damping; a zero stays zero),
+//                      the packed 2-bit minmod codes and the density floors
+//                      of get_Side;
+//   adj_face_compute   the flux VJP (HLLD with GLM for the plasma, HLLC for
+//                      the neutrals) and the floors' zeroing.
+template <int FL>
+struct FaceRaw {
+  static constexpr int NF = FL ? 5 : ADJ_NV;
+  double v[NF][4], mR[NF], mL[NF];
+};
+
+template <int FL>
+struct FaceIn {
+  static constexpr int NF = FL ? 5 : ADJ_NV;
+  double vl[NF], vr[NF], lamF[NF];
+  unsigned long long codes;
+  bool flL, flR;
+};
+
+template <int FL>
+PIP_HD void adj_face_raw(const AdjArgs& a, int dir, int fi, int fj, FaceRaw<FL>& r) {
+  constexpr int NF = FaceRaw<FL>::NF;
+  constexpr int OFF = FL ? ADJ_NV : 0;  // first plane of this fluid
+  const long long N = a.N;
+  const long long st = (dir == 0) ? 1 : a.ix;
+  const long long oR = (long long)fj * a.ix + fi;
+  const long long oL = oR - st;
+  const bool intL = (dir == 0) ? (fi - 1 >= 2) : (fj - 1 >= 2);
+  const bool intR = (dir == 0) ? (fi < a.ix - 2) : (fj < a.jx - 2);
+#pragma unroll
+  for (int n = 0; n < NF; n++) {
+    const int s = adj_perm(OFF + n, dir);
+    const double* p = a.V + s * N + oR;
+    r.v[n][0] = p[-2 * st];
+    r.v[n][1] = p[-st];
+    r.v[n][2] = p[0];
+    r.v[n][3] = p[st];
+    r.mR[n] = intR ? a.lam_out[s * N + oR] : 0.0;
+    r.mL[n] = intL ? a.lam_out[s * N + oL] : 0.0;
+  }
+}
+
+template <int FL>
+PIP_HD void adj_face_from_raw(const AdjArgs& a, int dir, const FaceRaw<FL>& r, FaceIn<FL>& in) {
+  constexpr int NF = FaceRaw<FL>::NF;
+  constexpr int OFF = FL ? ADJ_NV : 0;
+  double c0, c1, dt_sub, psi_fac;
+  adj_coeffs(a, c0, c1, dt_sub, psi_fac);
+  const double coef = dt_sub / ((dir == 0) ? a.dx : a.dy);
+  unsigned long long codes = 0ull;
+#pragma unroll
+  for (int n = 0; n < NF; n++) {
+    const int s = adj_perm(OFF + n, dir);
+    const double v0 = r.v[n][0], v1 = r.v[n][1], v2 = r.v[n][2], v3 = r.v[n][3];
+    in.vl[n] = v1 + 0.5 * minmod(v0, v1, v2);
+    in.vr[n] = v2 - 0.5 * minmod(v1, v2, v3);
+    in.lamF[n] = coef * (adj_mu_of(s, r.mR[n], psi_fac) - adj_mu_of(s, r.mL[n], psi_fac));
+    const unsigned cl = recon_code(v0, v1, v2);
+    const unsigned cr = recon_code(v1, v2, v3);
+    codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
+  }
+  in.codes = codes;
+  // density floor of get_Side (HLL_rot.f90:60-61)
+  in.flL = !(in.vl[0] > a.ph.ro_lim);
+  in.flR = !(in.vr[0] > a.ph.ro_lim);
+  in.vl[0] = dmax(in.vl[0], a.ph.ro_lim);
+  in.vr[0] = dmax(in.vr[0], a.ph.ro_lim);
+}
+
+template <int FL>
+PIP_HD void adj_face_compute(const AdjArgs& a, FaceIn<FL>& in, double* lvl, double* lvr) {
+  constexpr int NF = FaceIn<FL>::NF;
+#pragma unroll
+  for (int n = 0; n < NF; n++) { lvl[n] = 0.0; lvr[n] = 0.0; }
+  if (FL == 0) hlld9_flux_vjp(in.vl, in.vr, a.ph.gm, a.dtc[1], in.lamF, lvl, lvr);
+  else hllc_flux_vjp(in.vl, in.vr, a.ph.gm, in.lamF, lvl, lvr);
+  if (in.flL) lvl[0] = 0.0;
+  if (in.flR) lvr[0] = 0.0;
+}
+
 // The adjoints of the left and right states of the face with + cell
 // (fi, fj) in direction dir for the fluid FL (0: plasma, HLLD with GLM; 1:
 // neutrals, HLLC), density floor applied, into lvl / lvr (the fluid's
@@ -188,79 +274,52 @@ struct alignas(16) FacePair {
 // code_out before the flux VJP (nothing of the stencil stays live across it).
 template <int FL, class CodeOut>
 PIP_HD void adj_face_vjp(const AdjArgs& a, int dir, int fi, int fj, double* lvl, double* lvr, CodeOut code_out) {
-  constexpr int NV = ADJ_NV;
-  constexpr int NF = FL ? 5 : NV;   // variables of this fluid
-  constexpr int OFF = FL ? NV : 0;  // first plane of this fluid
-  const long long N = a.N;
-  const long long st = (dir == 0) ? 1 : a.ix;
-  const long long oR = (long long)fj * a.ix + fi;
-  const long long oL = oR - st;
-  double c0, c1, dt_sub, psi_fac;
-  adj_coeffs(a, c0, c1, dt_sub, psi_fac);
-  const bool intL = (dir == 0) ? (fi - 1 >= 2) : (fj - 1 >= 2);
-  const bool intR = (dir == 0) ? (fi < a.ix - 2) : (fj < a.jx - 2);
-  const double coef = dt_sub / ((dir == 0) ? a.dx : a.dy);
+  FaceIn<FL> in;
+  {
+    FaceRaw<FL> r;
+    adj_face_raw<FL>(a, dir, fi, fj, r);
+    adj_face_from_raw<FL>(a, dir, r, in);
+  }
+  code_out(in.codes);
+  adj_face_compute<FL>(a, in, lvl, lvr);
+}
 
-  double vl[NF], vr[NF], lamF[NF];
+// The pair planes (dir * NVT + n) at the face's + cell offset oR: the
+// adjoints of the left and right face states of variable n; adj_face_store
+// also writes the face's routing code.
+template <int FL>
+PIP_HD void adj_face_pairs(const AdjArgs& a, int dir, long long oR, const double* lvl, const double* lvr) {
+  constexpr int NF = FL ? 5 : ADJ_NV;
+  constexpr int OFF = FL ? ADJ_NV : 0;
+  FacePair* P = reinterpret_cast<FacePair*>(a.P) + (long long)dir * ADJ_NVT * a.N;
 #pragma unroll
   for (int n = 0; n < NF; n++) {
     const int s = adj_perm(OFF + n, dir);
-    const double* p = a.V + s * N + oR;
-    const double v0 = p[-2 * st], v1 = p[-st], v2 = p[0], v3 = p[st];
-    vl[n] = v1 + 0.5 * minmod(v0, v1, v2);
-    vr[n] = v2 - 0.5 * minmod(v1, v2, v3);
-    const double mR = intR ? adj_mu(a, s, oR, psi_fac) : 0.0;
-    const double mL = intL ? adj_mu(a, s, oL, psi_fac) : 0.0;
-    lamF[n] = coef * (mR - mL);
+    P[s * a.N + oR] = FacePair{lvl[n], lvr[n]};
   }
-  {
-    unsigned long long codes = 0ull;
-#pragma unroll
-    for (int n = 0; n < NF; n++) {
-      const int s = adj_perm(OFF + n, dir);
-      const double* p = a.V + s * N + oR;
-      const unsigned cl = recon_code(p[-2 * st], p[-st], p[0]);
-      const unsigned cr = recon_code(p[-st], p[0], p[st]);
-      codes |= (unsigned long long)(cl | (cr << 2)) << (4 * (s - OFF));
-    }
-    code_out(codes);
-  }
-  // density floor of get_Side (HLL_rot.f90:60-61)
-  const bool flL = !(vl[0] > a.ph.ro_lim), flR = !(vr[0] > a.ph.ro_lim);
-  vl[0] = dmax(vl[0], a.ph.ro_lim);
-  vr[0] = dmax(vr[0], a.ph.ro_lim);
-#pragma unroll
-  for (int n = 0; n < NF; n++) { lvl[n] = 0.0; lvr[n] = 0.0; }
-  if (FL == 0) hlld9_flux_vjp(vl, vr, a.ph.gm, a.dtc[1], lamF, lvl, lvr);
-  else hllc_flux_vjp(vl, vr, a.ph.gm, lamF, lvl, lvr);
-  if (flL) lvl[0] = 0.0;
-  if (flR) lvr[0] = 0.0;
+}
+template <int FL>
+PIP_HD void adj_face_store(const AdjArgs& a, int dir, long long oR, unsigned long long codes, const double* lvl,
+                           const double* lvr) {
+  adj_code_pairs(a, dir)[oR].c[FL] = codes;
+  adj_face_pairs<FL>(a, dir, oR, lvl, lvr);
 }
 
 // One thread per face and fluid; R = the cell on the + side of the face.
 // dir 0: R in i = 2..nx+2, j = 2..ny+1; dir 1: R in i = 2..nx+1, j = 2..ny+2.
-// Writes the pair plane (dir * NVT + n) at R: the adjoints of the left and
-// right face states of variable n, and the face's routing code.
+// Writes the face's pairs and routing code at R.
 template <int FL>
 PIP_HD void adj_face_item(const AdjArgs& a, int dir, long long t) {
-  constexpr int NVT = ADJ_NVT;
   constexpr int NF = FL ? 5 : ADJ_NV;
-  constexpr int OFF = FL ? ADJ_NV : 0;
   const int wx = (dir == 0) ? a.nx + 1 : a.nx;
   const int wy = (dir == 0) ? a.ny : a.ny + 1;
   if (t >= (long long)wx * wy) return;
   const int fi = (int)(t % wx) + 2, fj = (int)(t / wx) + 2;
-  const long long N = a.N;
   const long long oR = (long long)fj * a.ix + fi;
   double lvl[NF], lvr[NF];
   adj_face_vjp<FL>(a, dir, fi, fj, lvl, lvr,
                    [&](unsigned long long c) { adj_code_pairs(a, dir)[oR].c[FL] = c; });
-  FacePair* P = reinterpret_cast<FacePair*>(a.P) + (long long)dir * NVT * N;
-#pragma unroll
-  for (int n = 0; n < NF; n++) {
-    const int s = adj_perm(OFF + n, dir);
-    P[s * N + oR] = FacePair{lvl[n], lvr[n]};
-  }
+  adj_face_pairs<FL>(a, dir, oR, lvl, lvr);
 }
 
 // The neutral faces the gather reads at the ghost images of boundary cells:

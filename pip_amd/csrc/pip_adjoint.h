@@ -30,15 +30,16 @@ __device__ __forceinline__ int xcd_order(int bid, int nb) {
   return x * q + (x < rr ? x : rr) + k;
 }
 
-// The plasma's faces of both directions in one launch, 256 consecutive
-// faces (row-major) per workgroup: adj_face_blocks(nx+1, ny) workgroups take
-// the x faces, adj_face_blocks(nx, ny+1) the y faces.  Rows of nx+1 x faces
-// are not a multiple of 64, so rectangular tiles would leave waves with a
-// single live lane running the whole VJP.  The first adj_edge_face_blocks
-// workgroups compute the neutrals' boundary faces that the gather reads at
-// ghost images (adj_edge_face_item): dispatched first, they share the first
-// round with plasma workgroups instead of running alone after the last one
-// (7.7 us of the launch at 512^2).
+// The plasma's faces of both directions in one launch, in batches of 256
+// consecutive faces (row-major): adj_face_blocks(nx+1, ny) batches of x
+// faces, then adj_face_blocks(nx, ny+1) of y faces, ADJ_PF batches per
+// workgroup.  Rows of nx+1 x faces are not a multiple of 64, so rectangular
+// tiles would leave waves with a single live lane running the whole VJP.
+// The first adj_edge_face_blocks workgroups compute the neutrals' boundary
+// faces that the gather reads at ghost images (adj_edge_face_item):
+// dispatched first, they share the first round with plasma workgroups
+// instead of running alone after the last one (7.7 us of the launch at
+// 512^2).
 __host__ __device__ inline int adj_face_blocks(int w, int h) {
   return (int)(((long long)w * h + 255) / 256);
 }
@@ -49,21 +50,62 @@ __host__ __device__ inline int adj_edge_face_blocks(int nx, int ny) {
   return (int)((adj_edge_faces(nx, ny) + 255) / 256);
 }
 
-__global__ void __launch_bounds__(256, 1) k_adj_plasma_faces(AdjArgs a) {
+// Batch b of 256 plasma faces (b < adj_plasma_face_blocks): this thread's
+// face, as direction, + cell and its offset; false past the last face.
+__device__ __forceinline__ bool adj_plasma_face_of(const AdjArgs& a, int b, int& dir, int& fi, int& fj,
+                                                   long long& oR) {
   const int nbx = adj_face_blocks(a.nx + 1, a.ny);
-  const int nbf = adj_plasma_face_blocks(a.nx, a.ny);
+  if (b >= adj_plasma_face_blocks(a.nx, a.ny)) return false;
+  dir = (b < nbx) ? 0 : 1;
+  if (dir) b -= nbx;
+  const int w = dir ? a.nx : a.nx + 1, h = dir ? a.ny + 1 : a.ny;
+  const long long t = (long long)b * 256 + threadIdx.x;
+  if (t >= (long long)w * h) return false;
+  fi = (int)(t % w) + 2;
+  fj = (int)(t / w) + 2;
+  oR = (long long)fj * a.ix + fi;
+  return true;
+}
+
+// Each plasma workgroup takes two consecutive batches: a thread issues both
+// faces' stencil loads before the first face's VJP, so that the second
+// face's load latency passes under it (one wave per SIMD leaves no other
+// wave to hide it; three faces per thread fit in 475 registers but left a
+// longer last round and measured slower).
+constexpr int ADJ_PF = 2;
+__host__ __device__ inline int adj_plasma_face_groups(int nx, int ny) {
+  return (adj_plasma_face_blocks(nx, ny) + ADJ_PF - 1) / ADJ_PF;
+}
+
+__global__ void __launch_bounds__(256, 1) k_adj_plasma_faces(AdjArgs a) {
   const int nbe = adj_edge_face_blocks(a.nx, a.ny);
   if ((int)blockIdx.x < nbe) {
     adj_edge_face_item(a, (long long)blockIdx.x * 256 + threadIdx.x);
     return;
   }
-  int b = xcd_order((int)blockIdx.x - nbe, nbf);
-  const int dir = (b < nbx) ? 0 : 1;
-  if (dir) b -= nbx;
-  const int w = dir ? a.nx : a.nx + 1, h = dir ? a.ny + 1 : a.ny;
-  const long long t = (long long)b * 256 + threadIdx.x;
-  if (t >= (long long)w * h) return;
-  adj_face_item<0>(a, dir, t);
+  const int g = xcd_order((int)blockIdx.x - nbe, adj_plasma_face_groups(a.nx, a.ny));
+  int dir[ADJ_PF], fi[ADJ_PF], fj[ADJ_PF];
+  long long oR[ADJ_PF];
+  bool ok[ADJ_PF];
+  FaceRaw<0> raw[ADJ_PF];
+#pragma unroll
+  for (int q = 0; q < ADJ_PF; q++) {
+    dir[q] = fi[q] = fj[q] = 0;
+    oR[q] = 0;
+    ok[q] = adj_plasma_face_of(a, ADJ_PF * g + q, dir[q], fi[q], fj[q], oR[q]);
+    if (ok[q]) adj_face_raw<0>(a, dir[q], fi[q], fj[q], raw[q]);
+  }
+  FaceIn<0> in[ADJ_PF];
+#pragma unroll
+  for (int q = 0; q < ADJ_PF; q++)
+    if (ok[q]) adj_face_from_raw<0>(a, dir[q], raw[q], in[q]);
+#pragma unroll
+  for (int q = 0; q < ADJ_PF; q++) {
+    if (!ok[q]) continue;
+    double lvl[ADJ_NV], lvr[ADJ_NV];
+    adj_face_compute<0>(a, in[q], lvl, lvr);
+    adj_face_store<0>(a, dir[q], oR[q], in[q].codes, lvl, lvr);
+  }
 }
 
 // ---------------------------------------------------------------- gather
