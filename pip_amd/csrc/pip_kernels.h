@@ -829,17 +829,29 @@ __global__ void __launch_bounds__(256) k_bnd(double* U, int nvt, int ix, int jx,
 // End of cfl: combine the reduction slots into dt (Scheme_rot.f90:183-213).
 // mode 0: pip_cfl (optionally into the global dt); mode 2: the same at the
 // top of a fused loop (also sets active/last); mode 1: end of a fused step
-// (time += dt, stop_sim, next dt and cmax, activity flags).
+// (time += dt, stop_sim, next dt and cmax, activity flags).  In mode 1 a
+// recorded run also passes hist: the next step's {dt, cmax} go to
+// hist[2 hist_k], hist[2 hist_k + 1], as k_record_dt would write them.
 struct FinArgs {
   DevScalars* sc;
   int mode, into_global, flag_ir, flag_ir_type, flag_divb, pip_imp;
   double safety, dxyz_min, pip_imp_factor;
+  double* hist;
+  int hist_k;
 };
+
+__device__ __forceinline__ void record_dt(const DevScalars* sc, double* hist, int k) {
+  hist[2 * k] = sc->active ? sc->dt : 0.0;
+  hist[2 * k + 1] = sc->cmax;
+}
 
 __global__ void k_finalize(FinArgs f) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevScalars* sc = f.sc;
-  if (f.mode == 1 && !sc->active) return;
+  if (f.mode == 1 && !sc->active) {
+    if (f.hist) record_dt(sc, f.hist, f.hist_k);
+    return;
+  }
   if (f.mode == 1) sc->ion_pot_valid = 1;  // source_PIP ran in this step
   double dttemp = dmin(10.0, as_f64(sc->red[RED_DT]));
   // cfl_pn_col (Scheme_rot.f90:305-313)
@@ -885,6 +897,7 @@ __global__ void k_finalize(FinArgs f) {
     if (f.flag_divb >= 1) sc->cmax = f.safety * f.dxyz_min / sc->dt;
     sc->last = ((sc->time + sc->dt >= sc->tend) || (sc->nt + 1 >= sc->nmax) || (sc->dt <= 1.0e-12)) ? 1 : 0;
   }
+  if (f.hist) record_dt(sc, f.hist, f.hist_k);
 }
 
 }  // namespace PIP_NS
@@ -1425,8 +1438,7 @@ __global__ void k_lock_end(EnsArgs e, DevScalars* scs, const int* members, int c
 // = cmax of step k.
 __global__ void k_record_dt(const DevScalars* sc, double* hist, int k) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  hist[2 * k] = sc->active ? sc->dt : 0.0;
-  hist[2 * k + 1] = sc->cmax;
+  record_dt(sc, hist, k);
 }
 // End of a frozen-dt replay of n steps: the scalars a step-by-step run
 // leaves (time and step count advanced, dt / cmax of the last step).

@@ -309,9 +309,12 @@ static int launch_round(pip_ctx* c, const RoundArgs& r) {
   return PIP_OK;
 }
 
-static int launch_finalize(pip_ctx* c, int mode, int into_global) {
+// hist / hist_k (mode 1): record the next step's {dt, cmax} at hist_k.
+static int launch_finalize(pip_ctx* c, int mode, int into_global, double* hist = nullptr, int hist_k = 0) {
   FinArgs f;
   f.sc = c->sc;
+  f.hist = hist;
+  f.hist_k = hist_k;
   f.mode = mode;
   f.into_global = into_global;
   f.flag_ir = c->cfg.flag_ir;
@@ -933,10 +936,11 @@ int pip_record_steps(pip_ctx* c, int n, double* hist) {
   if (!c) return fail(PIP_ERR_ARG, "null context");
   if (n < 0 || (n > 0 && !hist)) return fail(PIP_ERR_ARG, "n < 0 or null history");
   if (c->top_pending) TRY(loop_top(c, 1.0e300, (long long)1 << 62));
+  // step 0's {dt, cmax} here, step k+1's by the end of step k
+  if (n > 0) hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, 0);
   for (int k = 0; k < n; k++) {
-    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
     TRY(run_stages(c, 2, nullptr));
-    TRY(launch_finalize(c, 1, 0));
+    TRY(launch_finalize(c, 1, 0, (k + 1 < n) ? hist : nullptr, k + 1));
   }
   HIPCHK(hipGetLastError());
   return PIP_OK;
@@ -958,18 +962,18 @@ int pip_record_steps_saved(pip_ctx* c, int n, double* hist, double* Un, double* 
   // each stage input saved by the stage kernels
   const size_t S = (size_t)c->N * c->nvt;
   HIPCHK(hipMemcpyAsync(Un, c->U, sizeof(double) * S, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, 0);
   for (int k = 0; k < n; k++) {
     double* un = Un + (size_t)k * S;
     double* u1 = U1 + (size_t)k * S;
     double* out = (k + 1 < n) ? Un + (size_t)(k + 1) * S : c->U;
-    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(64), 0, c->stream, c->sc, hist, k);
     StageArgs a1 = stage_args(c, 1, un, un, u1, 0, nullptr);
     StageArgs a2 = stage_args(c, 2, u1, un, out, 2, nullptr);
     a1.Vout = Vn + (size_t)k * S;
     a2.Vout = V1 + (size_t)k * S;
     TRY(launch_stage(c, a1));
     TRY(launch_stage(c, a2));
-    TRY(launch_finalize(c, 1, 0));
+    TRY(launch_finalize(c, 1, 0, (k + 1 < n) ? hist : nullptr, k + 1));
   }
   HIPCHK(hipGetLastError());
   return PIP_OK;
