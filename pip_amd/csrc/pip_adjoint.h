@@ -30,27 +30,30 @@ __device__ __forceinline__ int xcd_order(int bid, int nb) {
   return x * q + (x < rr ? x : rr) + k;
 }
 
-// The plasma's faces of both directions in one launch, in batches of 256
-// consecutive faces (row-major): adj_face_blocks(nx+1, ny) batches of x
-// faces, then adj_face_blocks(nx, ny+1) of y faces, ADJ_PF batches per
-// workgroup.  Rows of nx+1 x faces are not a multiple of 64, so rectangular
+// The plasma's faces of both directions in one launch, in batches of
+// ADJ_PBS consecutive faces (row-major): adj_face_blocks(nx+1, ny) batches
+// of x faces, then adj_face_blocks(nx, ny+1) of y faces, ADJ_PF batches per
+// workgroup of ADJ_PBS threads (two waves: at one wave per SIMD the launch
+// ends on finer units than with four-wave workgroups; 64 threads measured
+// slower).  Rows of nx+1 x faces are not a multiple of 64, so rectangular
 // tiles would leave waves with a single live lane running the whole VJP.
 // The first adj_edge_face_blocks workgroups compute the neutrals' boundary
 // faces that the gather reads at ghost images (adj_edge_face_item):
 // dispatched first, they share the first round with plasma workgroups
 // instead of running alone after the last one (7.7 us of the launch at
 // 512^2).
+constexpr int ADJ_PBS = 128;
 __host__ __device__ inline int adj_face_blocks(int w, int h) {
-  return (int)(((long long)w * h + 255) / 256);
+  return (int)(((long long)w * h + ADJ_PBS - 1) / ADJ_PBS);
 }
 __host__ __device__ inline int adj_plasma_face_blocks(int nx, int ny) {
   return adj_face_blocks(nx + 1, ny) + adj_face_blocks(nx, ny + 1);
 }
 __host__ __device__ inline int adj_edge_face_blocks(int nx, int ny) {
-  return (int)((adj_edge_faces(nx, ny) + 255) / 256);
+  return (int)((adj_edge_faces(nx, ny) + ADJ_PBS - 1) / ADJ_PBS);
 }
 
-// Batch b of 256 plasma faces (b < adj_plasma_face_blocks): this thread's
+// Batch b of ADJ_PBS plasma faces (b < adj_plasma_face_blocks): this thread's
 // face, as direction, + cell and its offset; false past the last face.
 __device__ __forceinline__ bool adj_plasma_face_of(const AdjArgs& a, int b, int& dir, int& fi, int& fj,
                                                    long long& oR) {
@@ -59,7 +62,7 @@ __device__ __forceinline__ bool adj_plasma_face_of(const AdjArgs& a, int b, int&
   dir = (b < nbx) ? 0 : 1;
   if (dir) b -= nbx;
   const int w = dir ? a.nx : a.nx + 1, h = dir ? a.ny + 1 : a.ny;
-  const long long t = (long long)b * 256 + threadIdx.x;
+  const long long t = (long long)b * ADJ_PBS + threadIdx.x;
   if (t >= (long long)w * h) return false;
   fi = (int)(t % w) + 2;
   fj = (int)(t / w) + 2;
@@ -77,10 +80,10 @@ __host__ __device__ inline int adj_plasma_face_groups(int nx, int ny) {
   return (adj_plasma_face_blocks(nx, ny) + ADJ_PF - 1) / ADJ_PF;
 }
 
-__global__ void __launch_bounds__(256, 1) k_adj_plasma_faces(AdjArgs a) {
+__global__ void __launch_bounds__(ADJ_PBS, 1) k_adj_plasma_faces(AdjArgs a) {
   const int nbe = adj_edge_face_blocks(a.nx, a.ny);
   if ((int)blockIdx.x < nbe) {
-    adj_edge_face_item(a, (long long)blockIdx.x * 256 + threadIdx.x);
+    adj_edge_face_item(a, (long long)blockIdx.x * ADJ_PBS + threadIdx.x);
     return;
   }
   const int g = xcd_order((int)blockIdx.x - nbe, adj_plasma_face_groups(a.nx, a.ny));

@@ -13,7 +13,7 @@ hipError_t adj_stage_launch(const AdjArgs& a, hipStream_t stream) {
   const dim3 cells((unsigned)((a.N + 255) / 256));
   if (a.make_V) hipLaunchKernelGGL(k_adj_cell, cells, dim3(256), 0, stream, a);
   const dim3 gf((unsigned)(adj_plasma_face_groups(a.nx, a.ny) + adj_edge_face_blocks(a.nx, a.ny)));
-  hipLaunchKernelGGL(k_adj_plasma_faces, gf, dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_adj_plasma_faces, gf, dim3(ADJ_PBS), 0, stream, a);
   const int ghost = (a.istep == 1 && !a.make_V) ? adj_ghost_blocks(a.ix, a.jx) : 0;
   hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)(adj_gather_tiles(a.nx, a.ny) + ghost)), dim3(256), 0, stream, a);
   return hipGetLastError();
