@@ -689,8 +689,8 @@ def main():
         write_pmc_summary(args.pmc_summary, cmds, args.workload, n, args.tile, arith)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
-              "multi-GPU sharding is a design, not a measured path", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU only (asked for {args.gpus}); sharding "
+              "across GPUs is designed, not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device plumbing only: sync around the timed region
